@@ -1,0 +1,208 @@
+/*
+ * tip.h -- C-ABI of libtip, the MI355X (gfx950) implementation of the 3D stochastic Langevin
+ * RK2 lattice step of thermal-inflation-phase-transition and its in-loop observables.
+ *
+ * Conventions
+ *  - Every entry point returns TIP_OK (0) or a negative TIP_ERR_* code; tip_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - Field pointers are DEVICE pointers (allocate with tip_malloc or any HIP allocator).
+ *    Fields are C-contiguous (nx, ny, nz) arrays, z fastest, exactly the reference's numpy
+ *    layout.  `stream` is a hipStream_t (NULL = the default stream); calls are asynchronous
+ *    with respect to the host unless stated otherwise.
+ *  - Scalars follow the reference's meaning and units (rescaled lattice units: dx = mu*dx_phys,
+ *    dt = mu*dt_phys, eta_eff, mu, inv_a2 = 1/a^2).
+ *  - Noise: a non-NULL `noise` array is added as the reference adds its pre-generated field
+ *    (pi += dt*k2 + noise).  With noise == NULL the library draws the reference's hash noise
+ *    inline: noise = noise_scale * z, z = sqrt(-2 ln u1) cos(2 pi u2), (u1, u2) from
+ *    _hash_rng_pair(uint64(site*73856093) ^ noise_seed).  The reference passes
+ *    noise_seed = uint64(step) (simulation/latticeSimeRescale_numba.py:3738); tip_noise_seed()
+ *    folds an ensemble seed into it so that seed 0 reproduces the reference stream.
+ *  - V' tables are float64 arrays of npts values on x_i = tmin + i/dinv, looked up exactly as
+ *    the reference does (truncating index clamped to [0, npts-2], unclamped linear fraction).
+ *    odd != 0 marks a table tabulated on x >= 0 that is read at |phi| with the sign restored.
+ */
+#ifndef TIP_H
+#define TIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The library is compiled with hidden visibility; only the declarations below are exported. */
+#if defined(TIP_BUILDING) && defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#define TIP_OK 0
+#define TIP_ERR_ARG -1      /* invalid argument (shape, NULL pointer, range) */
+#define TIP_ERR_HIP -2      /* a HIP runtime call failed */
+#define TIP_ERR_NOMEM -3    /* device allocation failed */
+#define TIP_ERR_STATE -4    /* object used in the wrong state */
+
+/* ---- library / memory plumbing ------------------------------------------------------- */
+int tip_version(int *major, int *minor);
+int tip_last_error(char *buf, size_t len);
+int tip_device_count(int *count);
+int tip_set_device(int device);
+int tip_malloc(void **ptr, size_t bytes);
+int tip_free(void *ptr);
+int tip_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int tip_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int tip_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int tip_memset(void *ptr, int value, size_t bytes, void *stream);
+int tip_stream_create(void **stream);
+int tip_stream_destroy(void *stream);
+int tip_stream_synchronize(void *stream);
+int tip_device_synchronize(void);
+
+/* Per-step noise seed: step ^ (seed * 0x9E3779B97F4A7C15).  seed = 0 gives uint64(step), the
+ * reference's noise_seed (simulation/latticeSimeRescale_numba.py:3738). */
+uint64_t tip_noise_seed(uint64_t step, uint64_t seed);
+
+/* ---- hash RNG (bit-exact with _hash_rng_pair) ----------------------------------------
+ * Replaces: _hash_rng_pair, simulation/latticeSimeRescale_numba.py:2322-2336
+ * (same text latticeSimComplex_numba.py:2747-2761).
+ * For n device seeds writes the mixer states x1, x2 and u1, u2 (u1 clamped at 1e-12), and
+ * z = sqrt(-2 ln u1) cos(2 pi u2), zs = ... sin(...) in float64.  Any output may be NULL. */
+int tip_hash_pairs(const uint64_t *seeds, int64_t n, uint64_t *x1, uint64_t *x2, double *u1,
+                   double *u2, double *z, double *zs, void *stream);
+
+/* Hash noise field of the fused kernels written to an array:
+ * noise1[site] = scale * r cos(2 pi u2), noise2 (nullable) = scale * r sin(2 pi u2),
+ * seed uint64(site*mult) ^ noise_seed.  mult = 73856093 (pass 2) or 19349669 (pass 4) in
+ * the reference (latticeSimeRescale_numba.py:2511-2522,2636-2647).
+ * Replaces: generate_noise_field(noise, scale, seed), latticeSimeRescale_numba.py:1348-1361,
+ * whose numpy per-thread stream is not reproducible; this is the deterministic stream. */
+int tip_noise_field_f64(double *noise1, double *noise2, int64_t nx, int64_t ny, int64_t nz,
+                        uint64_t noise_seed, int64_t mult, double scale, void *stream);
+int tip_noise_field_f32(float *noise1, float *noise2, int64_t nx, int64_t ny, int64_t nz,
+                        uint64_t noise_seed, int64_t mult, double scale, void *stream);
+
+/* ---- fused RK2 Langevin step (drop-in) -----------------------------------------------
+ * Replaces: rk2_step_table(phi, pi, dt, dx, eta_eff, mu, noise, phi_tmp, pi_tmp, inv_a2,
+ *           vp_table, vp_tmin, vp_dinv, vp_npts)
+ *           simulation/latticeSimeRescale_numba.py:1875-1966 (caller :3752-3768).
+ * phi and pi are updated in place; phi_tmp and pi_tmp are caller scratch (clobbered) as in
+ * the reference.  One fused kernel computes both RK2 stages from the 7-point stencil of
+ * (phi, pi), writes the result to (phi_tmp, pi_tmp), and the result is copied back.
+ * _f64: float64 fields and arithmetic in the reference's operation order.
+ * _f32: float32 fields, float32 arithmetic (float32 hash-noise transcendentals; u1, u2 exact).
+ * vac_count (nullable, device int64): += #sites with |phi_new| > vac_esc. */
+int tip_rk2_step_table_f64(double *phi, double *pi, double dt, double dx, double eta_eff,
+                           double mu, const double *noise, double *phi_tmp, double *pi_tmp,
+                           double inv_a2, const double *vp_table, double vp_tmin,
+                           double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                           int64_t nz, uint64_t noise_seed, double noise_scale,
+                           unsigned long long *vac_count, double vac_esc, void *stream);
+int tip_rk2_step_table_f32(float *phi, float *pi, double dt, double dx, double eta_eff,
+                           double mu, const float *noise, float *phi_tmp, float *pi_tmp,
+                           double inv_a2, const double *vp_table, double vp_tmin,
+                           double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                           int64_t nz, uint64_t noise_seed, double noise_scale,
+                           unsigned long long *vac_count, double vac_esc, void *stream);
+
+/* Out-of-place fused step (the performance form: ping-pong buffers, no copy back).
+ * (phi, pi) -> (phi_out, pi_out); the input and output buffers must not overlap.
+ * table_ab is a float32 segment table from tip_table_prepare_f32 (npts-1 float2 entries). */
+int tip_rk2_fused_f64(const double *phi, const double *pi, double *phi_out, double *pi_out,
+                      double dt, double dx, double eta_eff, double mu, double inv_a2,
+                      const double *noise, const double *vp_table, double vp_tmin,
+                      double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                      int64_t nz, uint64_t noise_seed, double noise_scale,
+                      unsigned long long *vac_count, double vac_esc, void *stream);
+int tip_rk2_fused_f32(const float *phi, const float *pi, float *phi_out, float *pi_out,
+                      double dt, double dx, double eta_eff, double mu, double inv_a2,
+                      const float *noise, const void *table_ab, double vp_tmin,
+                      double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                      int64_t nz, uint64_t noise_seed, double noise_scale,
+                      unsigned long long *vac_count, double vac_esc, void *stream);
+
+/* float64 table (device, npts values) -> float32 segment table (device, (npts-1) float2:
+ * V'(x) = A_i + B_i x on segment i).  Buffer size: 8*(npts-1) bytes. */
+int tip_table_prepare_f32(const double *vp_table, int64_t npts, double vp_tmin, double vp_dinv,
+                          void *table_ab, void *stream);
+
+/* ---- complex field (phi1 + i phi2) ---------------------------------------------------
+ * Replaces: rk2_step_table(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, noise2,
+ *           phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, inv_a2, vp_table, vp_tmin, vp_dinv, vp_npts)
+ *           simulation/latticeSimComplex_numba.py:2067-2215 (caller :4377-4398); the Z_N
+ *           parameters are the _ZN_PARAMS global (:227-231) passed explicitly.
+ * Inline hash noise (noise1 == NULL): one hash per site gives (cos, sin) components, as the
+ * reference's fused complex kernels (latticeSimComplex_numba.py:2961-2974).
+ * vac_count counts rho^2 > vac_esc^2. */
+int tip_rk2_step_table_c64(double *phi1, double *phi2, double *pi1, double *pi2, double dt,
+                           double dx, double eta_eff, double mu, const double *noise1,
+                           const double *noise2, double *phi1_tmp, double *phi2_tmp,
+                           double *pi1_tmp, double *pi2_tmp, double inv_a2,
+                           const double *vp_table, double vp_tmin, double vp_dinv,
+                           int64_t vp_npts, double zn_n, double zn_dv, double zn_active,
+                           int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
+                           double noise_scale, unsigned long long *vac_count, double vac_esc,
+                           void *stream);
+int tip_rk2_step_table_c32(float *phi1, float *phi2, float *pi1, float *pi2, double dt,
+                           double dx, double eta_eff, double mu, const float *noise1,
+                           const float *noise2, float *phi1_tmp, float *phi2_tmp,
+                           float *pi1_tmp, float *pi2_tmp, double inv_a2,
+                           const double *vp_table, double vp_tmin, double vp_dinv,
+                           int64_t vp_npts, double zn_n, double zn_dv, double zn_active,
+                           int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
+                           double noise_scale, unsigned long long *vac_count, double vac_esc,
+                           void *stream);
+int tip_rk2_fused_c32(const float *phi1, const float *phi2, const float *pi1, const float *pi2,
+                      float *phi1_out, float *phi2_out, float *pi1_out, float *pi2_out,
+                      double dt, double dx, double eta_eff, double mu, double inv_a2,
+                      const float *noise1, const float *noise2, const void *table_ab,
+                      double vp_tmin, double vp_dinv, int64_t vp_npts, double zn_n,
+                      double zn_dv, double zn_active, int64_t nx, int64_t ny, int64_t nz,
+                      uint64_t noise_seed, double noise_scale, unsigned long long *vac_count,
+                      double vac_esc, void *stream);
+
+/* ---- observables ---------------------------------------------------------------------
+ * Winding.  Replaces: compute_winding_number(phi1, phi2, winding_out),
+ * simulation/latticeSimComplex_numba.py:2683-2744 (numpy restatement tools/winding.py:26-64),
+ * and the string count int(sum(|W| > 0.5)) of :4661.
+ * Per site and plaquette orientation (0 = XY, 1 = XZ, 2 = YZ, corner order as the reference)
+ * the integer winding sum(wrap(dtheta))/2pi.  Outputs (all nullable, device):
+ *   plaq   int8 [3][nx*ny*nz]  per-plaquette windings
+ *   wsum   float [nx*ny*nz]    per-site sum (the reference's winding_out)
+ *   counts uint64 [2]          += {#sites with |wsum| > 0.5, #plaquettes with winding != 0}
+ * The integers are computed exactly from the signs of the float32 field (the phase
+ * difference crosses the branch cut iff the quadrant test and the sign of the exact float64
+ * cross product say so); they agree with the reference's float64 atan2 arithmetic except
+ * where a phase difference lies within rounding of +-pi. */
+int tip_winding_f32(const float *phi1, const float *phi2, int8_t *plaq, float *wsum,
+                    unsigned long long *counts, int64_t nx, int64_t ny, int64_t nz,
+                    void *stream);
+int tip_winding_f64(const double *phi1, const double *phi2, int8_t *plaq, float *wsum,
+                    unsigned long long *counts, int64_t nx, int64_t ny, int64_t nz,
+                    void *stream);
+
+/* True-vacuum count: += #sites with |phi| > esc (real) or phi1^2 + phi2^2 > esc^2 (complex).
+ * Replaces the off-line mean(|phi| > esc) of postprocess/compute_lattice_Tc1.py:24 and the
+ * falseVacuumFraction reduction of cosmolattice_ext/models/thermal_inflation.h:331-358. */
+int tip_vacuum_count_f32(const float *phi, int64_t n, double esc, unsigned long long *count,
+                         void *stream);
+int tip_vacuum_count_f64(const double *phi, int64_t n, double esc, unsigned long long *count,
+                         void *stream);
+int tip_vacuum_count_c32(const float *phi1, const float *phi2, int64_t n, double esc,
+                         unsigned long long *count, void *stream);
+
+/* min / max of a field (double[2] device output {min, max}); replaces the per-step host scans
+ * phi.min()/phi.max() of the table-range check, latticeSimeRescale_numba.py:3655-3656, and
+ * max(rho) of latticeSimComplex_numba.py:4268 (tip_rho_max_c32). */
+int tip_minmax_f32(const float *phi, int64_t n, double *out2, void *stream);
+int tip_minmax_f64(const double *phi, int64_t n, double *out2, void *stream);
+int tip_rho_max_c32(const float *phi1, const float *phi2, int64_t n, double *out1,
+                    void *stream);
+
+#if defined(TIP_BUILDING) && defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIP_H */

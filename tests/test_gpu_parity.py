@@ -1,0 +1,285 @@
+"""GPU parity of the HIP kernels against the CPU oracle (run on an MI355X: -m gpu).
+
+Bars (written into each test):
+  * hash stream: x1, x2, u1, u2 bit-exact; z within 4e-15 relative (libm difference only)
+  * float64 step: 100 RK2 steps at 64^3 with inline hash noise, field norms within 1e-12
+    relative of the oracle (the reference's own float64 semantics)
+  * float32 step: error against the float64 oracle stated and bounded
+  * winding: per-plaquette int8 windings and string counts bit-exact
+  * vacuum count, min/max: exact
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _tip():
+    import tip
+    from tip import kernels, kernels_complex
+    from tip.device import Counter, DeviceArray
+    return tip, kernels, kernels_complex, DeviceArray, Counter
+
+
+def c0_table(npts=8192):
+    """Config table: 8192 float64 points of V'(phi, T0=7000) on [0, 1.5 phi_true], odd."""
+    from tip import potential as P
+    p = P.ModelParams.real()
+    vt = P.thermal_vev(7000.0, p)
+    t, tmin, dinv = P.build_vprime_table(7000.0, 0.0, 1.5 * vt, npts, p)
+    return t, tmin, dinv
+
+
+def c0_params():
+    from tip import potential as P
+    mu, dx_phys, dt_phys, T0 = 1000.0, 0.002, 2.5e-4, 7000.0
+    dt, dx, eta = mu * dt_phys, mu * dx_phys, T0 / mu
+    ns = P.noise_scale(eta, T0, dt, mu, dx_phys)
+    return dict(dt=dt, dx=dx, eta_eff=eta, mu=mu, inv_a2=1.0, noise_scale=ns)
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# ----------------------------------------------------------------------------------------
+# hash RNG
+# ----------------------------------------------------------------------------------------
+def test_hash_bit_exact():
+    tip, K, _, _, _ = _tip()
+    edge = [0, 1, 2, 73856093, 0x9E3779B97F4A7C15, (1 << 63), (1 << 64) - 1, (1 << 32) - 1]
+    seeds = np.concatenate([np.array(edge, np.uint64), O.site_seeds(1 << 20, 12345)])
+    g = K.hash_rng_pairs(seeds)
+    x1, x2, u1, u2 = O.hash_pairs(seeds)
+    assert np.array_equal(g["x1"], x1) and np.array_equal(g["x2"], x2)
+    assert np.array_equal(g["u1"], u1) and np.array_equal(g["u2"], u2)
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = r * np.cos(6.283185307179586 * u2)
+    zs = r * np.sin(6.283185307179586 * u2)
+    assert np.max(np.abs(g["z"] - z) / np.maximum(np.abs(r), 1e-300)) < 4e-15
+    assert np.max(np.abs(g["zs"] - zs) / np.maximum(np.abs(r), 1e-300)) < 4e-15
+    assert K._hash_rng_pair(0) == (1e-12, float(u2[0]))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_noise_field_matches_oracle(dtype):
+    tip, K, _, DA, _ = _tip()
+    shape = (16, 24, 40)
+    for mult in (K.NOISE_MULT_PASS2, K.NOISE_MULT_PASS4):
+        a1, a2 = O.noise_field(shape, 7, 1750.0, dtype=np.float64, pair=True, mult=mult)
+        g1, g2 = np.empty(shape, dtype), np.empty(shape, dtype)
+        K.generate_noise_field(g1, 1750.0, 7, mult=mult, noise2=g2)
+        tol = 1e-14 if dtype == np.float64 else 2e-7
+        assert np.max(np.abs(g1 - a1)) < tol * 1750.0 * 8
+        assert np.max(np.abs(g2 - a2)) < tol * 1750.0 * 8
+
+
+# ----------------------------------------------------------------------------------------
+# real-field step
+# ----------------------------------------------------------------------------------------
+def _run_gpu_real(phi0, pi0, nsteps, prm, table, tmin, dinv, odd, dtype, seed=0):
+    tip, K, _, DA, Counter = _tip()
+    phi = DA.from_numpy(phi0.astype(dtype))
+    pi = DA.from_numpy(pi0.astype(dtype))
+    pt = DA(phi0.shape, dtype)
+    qt = DA(phi0.shape, dtype)
+    tab = DA.from_numpy(np.asarray(table, np.float64))
+    for n in range(nsteps):
+        K.rk2_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta_eff"], prm["mu"], None, pt, qt,
+                         prm["inv_a2"], tab, tmin, dinv, len(table), odd=odd,
+                         noise_seed=O.noise_seed(n, seed), noise_scale=prm["noise_scale"])
+    return phi.download(), pi.download()
+
+
+def test_f64_100_steps_c0_within_1e12():
+    """Config-0 shape (64^3 real), float64, 100 steps with hash noise: 1e-12 relative."""
+    rng = np.random.default_rng(2024)
+    shape = (64, 64, 64)
+    phi0 = 0.01 * rng.normal(size=shape)
+    pi0 = np.zeros(shape)
+    prm = c0_params()
+    t, tmin, dinv = c0_table()
+    print(f"[f64 c0] shape={shape} ns={prm['noise_scale']:.4g}", flush=True)
+    a, b = O.run_real(phi0, pi0, 100, prm, t, tmin, dinv, odd=True)
+    g, h = _run_gpu_real(phi0, pi0, 100, prm, t, tmin, dinv, True, np.float64)
+    e_phi, e_pi = rel(g, a), rel(h, b)
+    print(f"  rel err phi={e_phi:.3e} pi={e_pi:.3e}")
+    assert e_phi <= 1e-12 and e_pi <= 1e-12
+    # vacuum counts of the two runs agree exactly at a threshold inside the field's range
+    esc = float(np.quantile(np.abs(a), 0.9))
+    assert O.vacuum_count(g, esc) == O.vacuum_count(a, esc)
+
+
+@pytest.mark.parametrize("shape", [(5, 7, 9), (3, 70, 130), (1, 1, 1), (2, 3, 64), (33, 5, 65)])
+def test_f64_ragged_shapes_one_step_explicit_noise(shape):
+    """Explicit noise array (the reference's calling form) on ragged shapes: 1e-14."""
+    tip, K, _, DA, _ = _tip()
+    rng = np.random.default_rng(sum(shape))
+    phi = rng.normal(size=shape) * 3e3
+    pi = rng.normal(size=shape) * 1e3
+    noise = rng.normal(size=shape) * 500.0
+    t = rng.normal(size=257) * 1e8
+    tmin, dinv = -1e4, 256 / 2e4
+    a, b = phi.copy(), pi.copy()
+    O.rk2_step_table(a, b, 0.25, 2.0, 7.0, 1000.0, noise, np.empty_like(a), np.empty_like(b),
+                     0.9, t, tmin, dinv, len(t))
+    g, h = phi.copy(), pi.copy()
+    K.rk2_step_table(g, h, 0.25, 2.0, 7.0, 1000.0, noise, np.empty_like(g), np.empty_like(h), 0.9,
+                     t, tmin, dinv, len(t))
+    assert rel(g, a) < 1e-14 and rel(h, b) < 1e-14
+
+
+def test_f32_error_against_f64():
+    """float32 kernel vs float64 oracle after 100 steps (C0 shape): stated bound 1e-4."""
+    rng = np.random.default_rng(7)
+    shape = (64, 64, 64)
+    phi0 = 0.01 * rng.normal(size=shape)
+    pi0 = np.zeros(shape)
+    prm = c0_params()
+    t, tmin, dinv = c0_table()
+    a, b = O.run_real(phi0, pi0, 100, prm, t, tmin, dinv, odd=True)
+    g, h = _run_gpu_real(phi0, pi0, 100, prm, t, tmin, dinv, True, np.float32)
+    e_phi, e_pi = rel(g.astype(np.float64), a), rel(h.astype(np.float64), b)
+    print(f"[f32 vs f64, 100 steps] rel err phi={e_phi:.3e} pi={e_pi:.3e}")
+    assert e_phi < 1e-4 and e_pi < 1e-4
+
+
+def test_f32_matches_oracle_f32mixed_one_step():
+    """float32 kernel vs the oracle's float32-mixed semantics, one step, explicit noise."""
+    tip, K, _, DA, _ = _tip()
+    rng = np.random.default_rng(3)
+    shape = (16, 32, 64)
+    phi = (rng.normal(size=shape) * 3e3).astype(np.float32)
+    pi = (rng.normal(size=shape) * 1e3).astype(np.float32)
+    noise = (rng.normal(size=shape) * 500.0).astype(np.float32)
+    t = -1e6 * np.linspace(-1e5, 1e5, 1025)
+    tmin, dinv = -1e5, 1024 / 2e5
+    a, b = phi.copy(), pi.copy()
+    O.rk2_step_table(a, b, 0.25, 2.0, 7.0, 1000.0, noise, np.empty_like(a), np.empty_like(b),
+                     1.0, t, tmin, dinv, len(t))
+    g, h = phi.copy(), pi.copy()
+    K.rk2_step_table(g, h, 0.25, 2.0, 7.0, 1000.0, noise, np.empty_like(g), np.empty_like(h),
+                     1.0, t, tmin, dinv, len(t))
+    assert rel(g, a) < 1e-6 and rel(h, b) < 1e-6
+
+
+def test_fused_vacuum_count_matches_standalone():
+    tip, K, _, DA, Counter = _tip()
+    rng = np.random.default_rng(11)
+    shape = (32, 48, 64)
+    phi = DA.from_numpy((rng.normal(size=shape) * 2e4).astype(np.float32))
+    pi = DA.from_numpy((rng.normal(size=shape) * 1e3).astype(np.float32))
+    pt, qt = DA(shape, np.float32), DA(shape, np.float32)
+    t, tmin, dinv = -1e6 * np.linspace(-1e6, 1e6, 513), -1e6, 512 / 2e6
+    c = Counter(1)
+    K.rk2_step_table(phi, pi, 0.25, 2.0, 7.0, 1000.0, None, pt, qt, 1.0, t, tmin, dinv, len(t),
+                     noise_seed=3, noise_scale=100.0, vac_count=c, vac_esc=1e4)
+    fused = int(c.read()[0])
+    host = phi.download()
+    assert fused == int(np.sum(np.abs(host.astype(np.float64)) > 1e4))
+    assert fused == K.vacuum_count(phi, 1e4)
+    lo, hi = K.field_minmax(phi)
+    assert lo == float(host.min()) and hi == float(host.max())
+
+
+# ----------------------------------------------------------------------------------------
+# complex field
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("zn", [(0.0, 0.0, 0.0), (4.0, 3e8, 1.0)])
+def test_c64_20_steps(zn):
+    tip, K, KC, DA, _ = _tip()
+    rng = np.random.default_rng(5)
+    shape = (24, 20, 36)
+    f1, f2 = rng.normal(size=shape) * 1e3, rng.normal(size=shape) * 1e3
+    p1, p2 = np.zeros(shape), np.zeros(shape)
+    from tip import potential as P
+    p = P.ModelParams.complex()
+    t, tmin, dinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, p)
+    ns = P.noise_scale(1.0, 6770.0, 0.25, 1000.0, 1e-3)
+    a = [f1.copy(), f2.copy(), p1.copy(), p2.copy()]
+    tmp = [np.empty(shape) for _ in range(4)]
+    for n in range(20):
+        n1, n2 = O.noise_field(shape, O.noise_seed(n), ns, pair=True)
+        O.rk2_step_table_complex(*a, 0.25, 1.0, 1.0, 1000.0, n1, n2, *tmp, 1.0, t, tmin, dinv,
+                                 len(t), zn=zn)
+    d = [DA.from_numpy(x) for x in (f1, f2, p1, p2)]
+    dt_ = [DA(shape, np.float64) for _ in range(4)]
+    for n in range(20):
+        KC.rk2_step_table(*d, 0.25, 1.0, 1.0, 1000.0, None, None, *dt_, 1.0, t, tmin, dinv, len(t),
+                          zn=zn, noise_seed=O.noise_seed(n), noise_scale=ns)
+    g = [x.download() for x in d]
+    for gi, ai in zip(g, a):
+        assert rel(gi, ai) <= 1e-12
+
+
+def test_c32_vs_oracle_mixed():
+    tip, K, KC, DA, _ = _tip()
+    rng = np.random.default_rng(6)
+    shape = (16, 20, 64)
+    f = [(rng.normal(size=shape) * 1e3).astype(np.float32) for _ in range(2)]
+    p = [np.zeros(shape, np.float32) for _ in range(2)]
+    n1, n2 = O.noise_field(shape, 1, 300.0, dtype=np.float32, pair=True)
+    from tip import potential as P
+    t, tmin, dinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, P.ModelParams.complex())
+    a = [f[0].copy(), f[1].copy(), p[0].copy(), p[1].copy()]
+    O.rk2_step_table_complex(*a, 0.25, 1.0, 1.0, 1000.0, n1, n2,
+                             *[np.empty(shape, np.float32) for _ in range(4)], 1.0, t, tmin, dinv,
+                             len(t), zn=(3.0, 1e8, 1.0))
+    g = [f[0].copy(), f[1].copy(), p[0].copy(), p[1].copy()]
+    KC.rk2_step_table(*g, 0.25, 1.0, 1.0, 1000.0, n1, n2,
+                      *[np.empty(shape, np.float32) for _ in range(4)], 1.0, t, tmin, dinv, len(t),
+                      zn=(3.0, 1e8, 1.0))
+    for gi, ai in zip(g, a):
+        assert rel(gi, ai) < 1e-5
+
+
+# ----------------------------------------------------------------------------------------
+# winding, vacuum count
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(64, 64, 64), (7, 9, 70)])
+def test_winding_bit_exact(shape):
+    tip, K, KC, DA, _ = _tip()
+    rng = np.random.default_rng(shape[2])
+    p1 = rng.normal(size=shape).astype(np.float32)
+    p2 = rng.normal(size=shape).astype(np.float32)
+    w_ref, plaq_ref = O.winding(p1, p2)
+    w, nstr, nplq, plaq = KC.compute_winding_number(p1, p2, plaquettes=True)
+    assert np.array_equal(plaq, plaq_ref)
+    assert np.array_equal(w, np.rint(w_ref).astype(np.float32))
+    assert nstr == O.string_count(w_ref)
+    assert nplq == int(np.count_nonzero(plaq_ref))
+    # float64 inputs (the reference converts its float32 fields to float64)
+    w2, nstr2, _, plaq2 = KC.compute_winding_number(p1.astype(np.float64), p2.astype(np.float64),
+                                                    plaquettes=True)
+    assert np.array_equal(plaq2, plaq_ref) and nstr2 == nstr
+
+
+def test_winding_vortex_ring():
+    """A vortex ring: winding lives on a closed loop of plaquettes; counts match."""
+    tip, K, KC, DA, _ = _tip()
+    n = 32
+    i, j, k = np.meshgrid(*[np.arange(n, dtype=np.float64)] * 3, indexing="ij")
+    c = (n - 1) / 2.0
+    r = np.sqrt((i - c) ** 2 + (j - c) ** 2)
+    # phase winds around the circle r = 8 in the (r, z) half plane
+    p1 = (r - 8.0).astype(np.float32)
+    p2 = (k - c).astype(np.float32)
+    w_ref, plaq_ref = O.winding(p1, p2)
+    w, nstr, nplq, plaq = KC.compute_winding_number(p1, p2, plaquettes=True)
+    assert np.array_equal(plaq, plaq_ref)
+    assert nstr == O.string_count(w_ref) and nstr > 0
+
+
+def test_vacuum_counts_exact():
+    tip, K, KC, DA, _ = _tip()
+    rng = np.random.default_rng(9)
+    for dtype in (np.float32, np.float64):
+        phi = (rng.normal(size=(40, 50, 60)) * 1e4).astype(dtype)
+        assert K.vacuum_count(phi, 1e4) == O.vacuum_count(phi, 1e4)
+    a = (rng.normal(size=(40, 50, 60)) * 1e4).astype(np.float32)
+    b = (rng.normal(size=(40, 50, 60)) * 1e4).astype(np.float32)
+    assert K.vacuum_count(a, 1e4, b) == O.vacuum_count(a, 1e4, b)

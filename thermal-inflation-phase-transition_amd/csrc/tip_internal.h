@@ -1,0 +1,194 @@
+// tip_internal.h -- shared host/device helpers of libtip (MI355X / gfx950).
+//
+// Lattice layout in HBM: every field is one C-contiguous (Nx, Ny, Nz) array with z fastest,
+// site = (i*Ny + j)*Nz + k, as the reference's numpy arrays
+// (simulation/latticeSimeRescale_numba.py:1259-1260).  A wave of 64 lanes runs along z, so
+// every centre and y/x-neighbour load is one coalesced 256-byte (float) row segment.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+
+#include "../../include/tip.h"
+
+namespace tip {
+
+// ---------------------------------------------------------------------------------------
+// Error handling: every C-ABI entry point returns TIP_OK or a negative code and leaves a
+// message for tip_last_error().
+// ---------------------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int fail_hip(hipError_t e, const char* what);
+
+#define TIP_HIP(call)                                              \
+    do {                                                           \
+        hipError_t _e = (call);                                    \
+        if (_e != hipSuccess) return ::tip::fail_hip(_e, #call);   \
+    } while (0)
+
+#define TIP_REQUIRE(cond, ...)                                     \
+    do {                                                           \
+        if (!(cond)) {                                             \
+            ::tip::set_error(__VA_ARGS__);                         \
+            return TIP_ERR_ARG;                                    \
+        }                                                          \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Lattice dimensions are bounded so that a site index fits a signed 32-bit int inside the
+// kernels (1024^3 = 2^30 sites); the hash seed still uses the 64-bit site index.
+constexpr int64_t kMaxSites = int64_t(1) << 31;
+
+int check_dims(int64_t nx, int64_t ny, int64_t nz);
+
+// Library-owned device scratch, grown on demand (used by drop-in entry points that need a
+// converted table or a counter).  One per device.
+void* scratch_get(size_t bytes, int slot);
+
+// ---------------------------------------------------------------------------------------
+// Counter-based hash RNG of the reference (simulation/latticeSimeRescale_numba.py:2322-2336).
+// x1 = state after the first mixer, x2 after the second.  u = (x >> 11) * 2^-53 is exact in
+// float64.  u1 is clamped below at 1e-12 as in the reference.
+// ---------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ void hash_states(uint64_t seed, uint64_t& x1, uint64_t& x2)
+{
+    uint64_t x = seed;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    x = x ^ (x >> 31);
+    x1 = x;
+    x = (x * 0x2545F4914F6CDD1DULL) ^ 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = x ^ (x >> 31);
+    x2 = x;
+}
+
+__host__ __device__ __forceinline__ double hash_unit(uint64_t x)
+{
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Site seed of the fused reference kernels: uint64(site*K) ^ uint64(noise_seed), K = 73856093
+// for the full-step noise (simulation/latticeSimeRescale_numba.py:2514-2516).
+constexpr int64_t kNoiseMult = 73856093;
+
+__host__ __device__ __forceinline__ uint64_t site_seed(int64_t site, uint64_t noise_seed)
+{
+    return (uint64_t)(site * kNoiseMult) ^ noise_seed;
+}
+
+// float64 Box-Muller exactly as written in the reference (cos for the first component,
+// sin for the second of the complex pair, latticeSimComplex_numba.py:2965-2974).
+__device__ __forceinline__ double gauss_f64(uint64_t seed)
+{
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    double u1 = hash_unit(x1), u2 = hash_unit(x2);
+    if (u1 < 1e-12) u1 = 1e-12;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double& zs)
+{
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    double u1 = hash_unit(x1), u2 = hash_unit(x2);
+    if (u1 < 1e-12) u1 = 1e-12;
+    double r = sqrt(-2.0 * log(u1));
+    double a = 6.283185307179586 * u2;
+    double s, c;
+    sincos(a, &s, &c);
+    zc = r * c;
+    zs = r * s;
+}
+
+// float32 Box-Muller from the same exact (u1, u2): u's rounded to float32, then the hardware
+// transcendentals.  v_cos/v_sin take their argument in revolutions, so cos(2*pi*u2) is one
+// instruction; -2 ln u1 = -2 ln2 * log2 u1.
+__device__ __forceinline__ void units_f32(uint64_t seed, float& u1f, float& u2f)
+{
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    double u1 = hash_unit(x1);
+    if (u1 < 1e-12) u1 = 1e-12;
+    u1f = (float)u1;
+    u2f = (float)hash_unit(x2);
+}
+
+__device__ __forceinline__ float gauss_f32(uint64_t seed)
+{
+    float u1f, u2f;
+    units_f32(seed, u1f, u2f);
+    float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1f));
+    return r * __builtin_amdgcn_cosf(u2f);
+}
+
+__device__ __forceinline__ void gauss_pair_f32(uint64_t seed, float& zc, float& zs)
+{
+    float u1f, u2f;
+    units_f32(seed, u1f, u2f);
+    float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1f));
+    zc = r * __builtin_amdgcn_cosf(u2f);
+    zs = r * __builtin_amdgcn_sinf(u2f);
+}
+
+// ---------------------------------------------------------------------------------------
+// V' tables.
+//
+// TableF64: the reference lookup verbatim (latticeSimeRescale_numba.py:1923-1930): truncating
+// int(), index clamped to [0, npts-2], frac unclamped (linear extrapolation).  odd != 0 reads
+// a table tabulated on x >= 0 at |x| and restores the sign.
+//
+// TableF32: the same piecewise-linear function stored per segment as (A_i, B_i) with
+// V'(x) = A_i + B_i x, prepared in float64 from the float64 table.  Evaluating in this form
+// keeps float32 accurate where x is small next to the table origin (x - tmin would cancel).
+// ---------------------------------------------------------------------------------------
+struct TableF64 {
+    const double* t;
+    double tmin, dinv;
+    int last;  // npts - 2
+    int odd;
+    __device__ __forceinline__ double operator()(double x) const
+    {
+        double sgn = 1.0;
+        if (odd && x < 0.0) { x = -x; sgn = -1.0; }
+        double fidx = (x - tmin) * dinv;
+        // Clamping fidx to [-1, last+1] before the conversion changes idx only where the
+        // index clamp below would anyway, and keeps the conversion in range (NaN -> 0).
+        int idx = (int)fmin(fmax(fidx, -1.0), (double)last + 1.0);
+        if (idx < 0) idx = 0;
+        else if (idx > last) idx = last;
+        double frac = fidx - (double)idx;
+        double t0 = t[idx], t1 = t[idx + 1];
+        double v = t0 + frac * (t1 - t0);
+        return odd ? sgn * v : v;
+    }
+};
+
+struct TableF32 {
+    const float2* ab;
+    float tmin, dinv;
+    int last;
+    int odd;
+    __device__ __forceinline__ float operator()(float x) const
+    {
+        float ax = odd ? fabsf(x) : x;
+        float fidx = (ax - tmin) * dinv;
+        int idx = (int)fminf(fmaxf(fidx, -1.0f), (float)last + 1.0f);
+        idx = idx < 0 ? 0 : (idx > last ? last : idx);
+        float2 s = ab[idx];
+        float v = fmaf(s.y, ax, s.x);
+        return (odd && x < 0.0f) ? -v : v;
+    }
+};
+
+// Convert a float64 reference table into float32 (A, B) segments on the device.
+int prepare_table_f32(const double* table_dev, int64_t npts, double tmin, double dinv,
+                      float2* ab_dev, hipStream_t s);
+
+// Lookup on the host of one segment table built from a host table (used for argument checks).
+int check_table(const void* table, int64_t npts, double dinv);
+
+}  // namespace tip

@@ -1,0 +1,227 @@
+// tip_step_f32.hip -- float32 fused RK2 step (the performance path) and the float32 segment
+// table it reads.
+#include "step_kernels.h"
+
+using namespace tip;
+
+namespace tip {
+
+// Segment i of the reference's piecewise-linear lookup (latticeSimeRescale_numba.py:1923-1930)
+// as V'(x) = A_i + B_i x: B_i = (t[i+1]-t[i]) * dinv, A_i = t[i] - B_i * (tmin + i/dinv),
+// computed in float64 and rounded once.
+__global__ void k_prepare_table_f32(const double* __restrict__ t, int nseg, double tmin,
+                                    double dinv, float2* __restrict__ ab)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    double b = (t[i + 1] - t[i]) * dinv;
+    double xi = tmin + (double)i / dinv;
+    double a = t[i] - b * xi;
+    ab[i] = make_float2((float)a, (float)b);
+}
+
+int prepare_table_f32(const double* table_dev, int64_t npts, double tmin, double dinv,
+                      float2* ab_dev, hipStream_t s)
+{
+    TIP_REQUIRE(table_dev && ab_dev, "table_prepare_f32: NULL pointer");
+    TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "table_prepare_f32: npts out of range");
+    TIP_REQUIRE(dinv > 0.0 && dinv < 1e300, "table_prepare_f32: dinv must be positive");
+    int nseg = (int)(npts - 1);
+    hipLaunchKernelGGL(k_prepare_table_f32, dim3((nseg + 255) / 256), dim3(256), 0, s, table_dev,
+                       nseg, tmin, dinv, ab_dev);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field, int f64)
+{
+    (void)complex_field;
+    (void)f64;
+    StepGeometry g;
+    g.by = 4;
+    int64_t per_plane = ((nz + 63) / 64) * ((ny + 3) / 4);
+    // Aim for about 4096 blocks (16 per CU) while marching at least 1 and at most 32 planes.
+    int64_t cx = (nx * per_plane) / 4096;
+    if (cx < 1) cx = 1;
+    if (cx > 32) cx = 32;
+    g.cx = (int)cx;
+    return g;
+}
+
+}  // namespace tip
+
+namespace {
+
+TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd)
+{
+    TableF32 tab;
+    tab.ab = reinterpret_cast<const float2*>(ab);
+    tab.tmin = (float)tmin;
+    tab.dinv = (float)dinv;
+    tab.last = (int)(npts - 2);
+    tab.odd = odd ? 1 : 0;
+    return tab;
+}
+
+int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out, double dt,
+                    double dx, double eta_eff, double mu, double inv_a2, const float* noise,
+                    const void* ab, double vp_tmin, double vp_dinv, int64_t vp_npts, int vp_odd,
+                    int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                    unsigned long long* vac, double vac_esc, hipStream_t st)
+{
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(phi && pi && phi_out && pi_out && ab, "rk2 f32: NULL field or table");
+    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 f32: vp_npts out of range");
+    TIP_REQUIRE(phi != phi_out && pi != pi_out, "rk2 f32: output must not alias input");
+    StepScalars<float> c = make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale);
+    StepGeometry g = step_geometry(nx, ny, nz, 0, 0);
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
+              (unsigned)((nx + g.cx - 1) / g.cx));
+    int hash = (noise == nullptr && noise_scale != 0.0) ? 1 : 0;
+    hipLaunchKernelGGL((k_rk2_real<float, TableF32, 4>), grid, block, 0, st, phi, pi, phi_out,
+                       pi_out, noise, c, make_table32(ab, vp_tmin, vp_dinv, vp_npts, vp_odd),
+                       (int)nx, (int)ny, (int)nz, g.cx, noise_seed, hash, vac, (float)vac_esc);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_table_prepare_f32(const double* vp_table, int64_t npts, double vp_tmin,
+                                     double vp_dinv, void* table_ab, void* stream)
+{
+    return prepare_table_f32(vp_table, npts, vp_tmin, vp_dinv,
+                             reinterpret_cast<float2*>(table_ab), as_stream(stream));
+}
+
+extern "C" int tip_rk2_fused_f32(const float* phi, const float* pi, float* phi_out, float* pi_out,
+                                 double dt, double dx, double eta_eff, double mu, double inv_a2,
+                                 const float* noise, const void* table_ab, double vp_tmin,
+                                 double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx,
+                                 int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                                 unsigned long long* vac_count, double vac_esc, void* stream)
+{
+    return launch_real_f32(phi, pi, phi_out, pi_out, dt, dx, eta_eff, mu, inv_a2, noise, table_ab,
+                           vp_tmin, vp_dinv, vp_npts, vp_odd, nx, ny, nz, noise_seed, noise_scale,
+                           vac_count, vac_esc, as_stream(stream));
+}
+
+extern "C" int tip_rk2_step_table_f32(float* phi, float* pi, double dt, double dx,
+                                      double eta_eff, double mu, const float* noise,
+                                      float* phi_tmp, float* pi_tmp, double inv_a2,
+                                      const double* vp_table, double vp_tmin, double vp_dinv,
+                                      int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                                      int64_t nz, uint64_t noise_seed, double noise_scale,
+                                      unsigned long long* vac_count, double vac_esc,
+                                      void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    TIP_REQUIRE(phi_tmp && pi_tmp, "rk2_step_table_f32: phi_tmp/pi_tmp are required");
+    TIP_REQUIRE(vp_table, "rk2_step_table_f32: NULL table");
+    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 f32: vp_npts out of range");
+    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
+    TIP_REQUIRE(ab != nullptr, "rk2_step_table_f32: scratch allocation failed");
+    int rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
+    if (rc) return rc;
+    rc = launch_real_f32(phi, pi, phi_tmp, pi_tmp, dt, dx, eta_eff, mu, inv_a2, noise, ab, vp_tmin,
+                         vp_dinv, vp_npts, vp_odd, nx, ny, nz, noise_seed, noise_scale, vac_count,
+                         vac_esc, st);
+    if (rc) return rc;
+    size_t bytes = (size_t)(nx * ny * nz) * sizeof(float);
+    TIP_HIP(hipMemcpyAsync(phi, phi_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    TIP_HIP(hipMemcpyAsync(pi, pi_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    return TIP_OK;
+}
+
+namespace {
+
+ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active)
+{
+    ZnParams<float> zn;
+    zn.n = (float)zn_n;
+    zn.dv = (float)zn_dv;
+    zn.active = (zn_active > 0.5 && zn_n > 0.5) ? 1 : 0;
+    double r = zn_n - (double)(int64_t)zn_n;
+    zn.ipow = (r == 0.0 && zn_n >= 1.0 && zn_n <= 32.0) ? (int)zn_n : 0;
+    return zn;
+}
+
+int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
+                       float* o1, float* o2, float* o3, float* o4, double dt, double dx,
+                       double eta_eff, double mu, double inv_a2, const float* noise1,
+                       const float* noise2, const void* ab, double vp_tmin, double vp_dinv,
+                       int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
+                       int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                       unsigned long long* vac, double vac_esc, hipStream_t st)
+{
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && ab,
+                "rk2 c32: NULL field or table");
+    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 c32: vp_npts out of range");
+    TIP_REQUIRE(noise1 != nullptr || noise2 == nullptr, "rk2 c32: noise2 without noise1");
+    StepScalars<float> c = make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale);
+    StepGeometry g = step_geometry(nx, ny, nz, 1, 0);
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
+              (unsigned)((nx + g.cx - 1) / g.cx));
+    int hash = (noise1 == nullptr && noise_scale != 0.0) ? 1 : 0;
+    hipLaunchKernelGGL((k_rk2_complex<float, TableF32, 4>), grid, block, 0, st, phi1, phi2, pi1,
+                       pi2, o1, o2, o3, o4, noise1, noise2, c,
+                       make_table32(ab, vp_tmin, vp_dinv, vp_npts, 0),
+                       make_zn32(zn_n, zn_dv, zn_active), (int)nx, (int)ny, (int)nz, g.cx,
+                       noise_seed, hash, vac, (float)(vac_esc * vac_esc));
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_rk2_fused_c32(const float* phi1, const float* phi2, const float* pi1,
+                                 const float* pi2, float* phi1_out, float* phi2_out,
+                                 float* pi1_out, float* pi2_out, double dt, double dx,
+                                 double eta_eff, double mu, double inv_a2, const float* noise1,
+                                 const float* noise2, const void* table_ab, double vp_tmin,
+                                 double vp_dinv, int64_t vp_npts, double zn_n, double zn_dv,
+                                 double zn_active, int64_t nx, int64_t ny, int64_t nz,
+                                 uint64_t noise_seed, double noise_scale,
+                                 unsigned long long* vac_count, double vac_esc, void* stream)
+{
+    TIP_REQUIRE(phi1 != phi1_out && phi2 != phi2_out && pi1 != pi1_out && pi2 != pi2_out,
+                "rk2 c32: output must not alias input");
+    return launch_complex_f32(phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out, pi2_out, dt, dx,
+                              eta_eff, mu, inv_a2, noise1, noise2, table_ab, vp_tmin, vp_dinv,
+                              vp_npts, zn_n, zn_dv, zn_active, nx, ny, nz, noise_seed,
+                              noise_scale, vac_count, vac_esc, as_stream(stream));
+}
+
+extern "C" int tip_rk2_step_table_c32(
+    float* phi1, float* phi2, float* pi1, float* pi2, double dt, double dx, double eta_eff,
+    double mu, const float* noise1, const float* noise2, float* phi1_tmp, float* phi2_tmp,
+    float* pi1_tmp, float* pi2_tmp, double inv_a2, const double* vp_table, double vp_tmin,
+    double vp_dinv, int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
+    int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+    unsigned long long* vac_count, double vac_esc, void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    TIP_REQUIRE(phi1_tmp && phi2_tmp && pi1_tmp && pi2_tmp, "rk2 c32: tmp arrays are required");
+    TIP_REQUIRE(vp_table, "rk2 c32: NULL table");
+    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 c32: vp_npts out of range");
+    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
+    TIP_REQUIRE(ab != nullptr, "rk2 c32: scratch allocation failed");
+    int rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
+    if (rc) return rc;
+    rc = launch_complex_f32(phi1, phi2, pi1, pi2, phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, dt, dx,
+                            eta_eff, mu, inv_a2, noise1, noise2, ab, vp_tmin, vp_dinv, vp_npts,
+                            zn_n, zn_dv, zn_active, nx, ny, nz, noise_seed, noise_scale,
+                            vac_count, vac_esc, st);
+    if (rc) return rc;
+    size_t bytes = (size_t)(nx * ny * nz) * sizeof(float);
+    TIP_HIP(hipMemcpyAsync(phi1, phi1_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    TIP_HIP(hipMemcpyAsync(phi2, phi2_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    TIP_HIP(hipMemcpyAsync(pi1, pi1_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    TIP_HIP(hipMemcpyAsync(pi2, pi2_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    return TIP_OK;
+}

@@ -1,0 +1,121 @@
+"""ctypes binding of libtip.so (the C-ABI declared in include/tip.h).
+
+The shared library is built in-tree by build.py and sits next to this file.  There is no
+fallback: if the library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("TIP_LIBRARY", _HERE / "libtip.so"))
+
+_vp = C.c_void_p
+_d = C.c_double
+_i64 = C.c_int64
+_u64 = C.c_uint64
+_int = C.c_int
+_sz = C.c_size_t
+_pint = C.POINTER(C.c_int)
+_ppv = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes); every symbol of include/tip.h
+SIGNATURES = {
+    "tip_version": (_int, [_pint, _pint]),
+    "tip_last_error": (_int, [C.c_char_p, _sz]),
+    "tip_device_count": (_int, [_pint]),
+    "tip_set_device": (_int, [_int]),
+    "tip_malloc": (_int, [_ppv, _sz]),
+    "tip_free": (_int, [_vp]),
+    "tip_memcpy_h2d": (_int, [_vp, _vp, _sz, _vp]),
+    "tip_memcpy_d2h": (_int, [_vp, _vp, _sz, _vp]),
+    "tip_memcpy_d2d": (_int, [_vp, _vp, _sz, _vp]),
+    "tip_memset": (_int, [_vp, _int, _sz, _vp]),
+    "tip_stream_create": (_int, [_ppv]),
+    "tip_stream_destroy": (_int, [_vp]),
+    "tip_stream_synchronize": (_int, [_vp]),
+    "tip_device_synchronize": (_int, []),
+    "tip_noise_seed": (_u64, [_u64, _u64]),
+    "tip_hash_pairs": (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tip_noise_field_f64": (_int, [_vp, _vp, _i64, _i64, _i64, _u64, _i64, _d, _vp]),
+    "tip_noise_field_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _u64, _i64, _d, _vp]),
+    "tip_rk2_step_table_f64": (_int, [
+        _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _d, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
+        _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_step_table_f32": (_int, [
+        _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _d, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
+        _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_fused_f64": (_int, [
+        _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
+        _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_fused_f32": (_int, [
+        _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
+        _u64, _d, _vp, _d, _vp]),
+    "tip_table_prepare_f32": (_int, [_vp, _i64, _d, _d, _vp, _vp]),
+    "tip_rk2_step_table_c64": (_int, [
+        _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _d, _d, _i64,
+        _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_step_table_c32": (_int, [
+        _vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _d, _d, _i64,
+        _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_fused_c32": (_int, [
+        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp, _vp, _d, _d, _i64,
+        _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp, _d, _vp]),
+    "tip_winding_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "tip_winding_f64": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "tip_vacuum_count_f32": (_int, [_vp, _i64, _d, _vp, _vp]),
+    "tip_vacuum_count_f64": (_int, [_vp, _i64, _d, _vp, _vp]),
+    "tip_vacuum_count_c32": (_int, [_vp, _vp, _i64, _d, _vp, _vp]),
+    "tip_minmax_f32": (_int, [_vp, _i64, _vp, _vp]),
+    "tip_minmax_f64": (_int, [_vp, _i64, _vp, _vp]),
+    "tip_rho_max_c32": (_int, [_vp, _vp, _i64, _vp, _vp]),
+}
+
+
+class TipError(RuntimeError):
+    """A libtip entry point returned an error code."""
+
+    def __init__(self, name: str, code: int, msg: str):
+        super().__init__(f"{name} -> {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libtip.so and declare every signature.  Raises if the library is absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"libtip.so not found at {LIB_PATH}; build it with "
+                "`python thermal-inflation-phase-transition_amd/build.py`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    load().tip_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name: str, *args) -> None:
+    """Call an int-returning entry point and raise TipError on failure."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise TipError(name, rc, last_error())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().tip_device_count(C.byref(n))
+    return int(n.value) if rc == 0 else 0
