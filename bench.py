@@ -1,0 +1,187 @@
+"""Benchmark of the fused RK2 Langevin lattice step on one MI355X.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload real256] [--no-cpu]
+
+Workload (default "real256"): real scalar field on a 256^3 periodic lattice, float32,
+V' table of 8192 float64 points on [0, 1.5 phi_true] (odd) built at T0 = 7000 GeV from the
+tree + thermal potential (lambda = 1e-24, V_correct, nb = nf = 20, y = 1.09, g = 1.05),
+dx = 2.0 (dx_phys = 0.002), dt = 0.25 (dt_phys = 2.5e-4), eta = T0/mu = 7, hash noise with
+the fluctuation-dissipation amplitude; phi0 = 0.01 N(0, 1), pi0 = 0.  One step = one RK2
+step over the whole lattice = one fused kernel launch (state resident in HBM, ping-pong).
+
+Prints one JSON line: site-updates/s (value), HBM GB/s at the algorithmic 16 B/site, the
+roofline of the fused kernel (HIP-event time on the launch stream) and the CPU baseline (the
+oracle's float32 two-pass restatement of rk2_step_table with its separate hash-noise pass,
+on all host cores, over a bounded sample of steps).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "thermal-inflation-phase-transition_amd"
+for _p in (str(ROOT), str(PKG)):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBPS = 8000.0
+REF_CPU_SITE_UPDATES = 2.21e8  # reference log, rk2_step_table 256^3 float32, 11 threads
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def real_workload(n: int = 256, npts: int = 8192):
+    from tip import potential as P
+
+    p = P.ModelParams.real()
+    T0, mu, dx_phys, dt_phys = 7000.0, 1000.0, 0.002, 2.5e-4
+    vt = P.thermal_vev(T0, p)
+    table, tmin, dinv = P.build_vprime_table(T0, 0.0, 1.5 * vt, npts, p)
+    dt, dx, eta = mu * dt_phys, mu * dx_phys, T0 / mu
+    prm = dict(dt=dt, dx=dx, eta_eff=eta, mu=mu, inv_a2=1.0,
+               noise_scale=P.noise_scale(eta, T0, dt, mu, dx_phys))
+    return table, tmin, dinv, prm
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch recorded by the committed rocprofv3 --pmc summary, if present."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(shape, table, tmin, dinv, prm, budget_s: float = 12.0):
+    """Oracle float32 restatement of rk2_step_table (2 passes) + hash noise pass, timed."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(1)
+    phi = (0.01 * rng.normal(size=shape)).astype(np.float32)
+    pi = np.zeros(shape, np.float32)
+    pt, qt = np.empty_like(phi), np.empty_like(pi)
+    noise = np.empty(shape, np.float32)
+
+    def one(n):
+        O.lib().orc_noise_field_f32(noise, None, *shape, O.noise_seed(n), 73856093,
+                                    prm["noise_scale"])
+        O.rk2_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta_eff"], prm["mu"], noise, pt, qt,
+                         prm["inv_a2"], table, tmin, dinv, len(table), odd=True)
+
+    one(0)  # warm-up (page faults, thread pool)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        one(steps + 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    sites = float(np.prod(shape))
+    return dict(value=sites * steps / el, unit="site-updates/s", cores=O.num_threads(),
+                kind="port",
+                sample=(f"{steps} steps of {shape[0]}x{shape[1]}x{shape[2]} real float32: oracle "
+                        f"rk2_step_table (2 OpenMP passes, float32 fields, float64 arithmetic) "
+                        f"+ hash noise pass, {el:.1f} s"),
+                ms_per_step=1e3 * el / steps)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="real256", choices=["real256", "real256_f64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the "
+              "multi-GPU sharding is described in DESIGN.md, not implemented.", file=sys.stderr)
+        return 2
+
+    import tip
+    from tip.lattice import Lattice
+
+    if tip.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    n = 256
+    shape = (n, n, n)
+    dtype = np.float64 if a.workload == "real256_f64" else np.float32
+    table, tmin, dinv, prm = real_workload(n)
+    log(f"workload {a.workload}: {shape} {np.dtype(dtype).name}, table {len(table)} pts, "
+        f"ns={prm['noise_scale']:.4g}")
+    lat = Lattice(shape, dtype)
+    rng = np.random.default_rng(0)
+    lat.upload("phi", 0.01 * rng.normal(size=shape))
+    lat.upload("pi", np.zeros(shape))
+    lat.set_table(table, tmin, dinv, odd=True)
+    kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+              inv_a2=prm["inv_a2"], noise_scale=prm["noise_scale"])
+    lat.step(a.warmup, timed=True, **kw)
+    log(f"warmup {a.warmup} steps done")
+    t0 = time.perf_counter()
+    ev_ms = lat.step(a.steps, timed=True, **kw)  # synchronous (event sync) on return
+    wall = time.perf_counter() - t0
+    log(f"timed {a.steps} steps: wall {wall * 1e3:.2f} ms, events {ev_ms:.2f} ms")
+    frac_true = lat.vacuum_fraction(1e4)
+    lo, hi = lat.minmax()
+    if not (math.isfinite(lo) and math.isfinite(hi)):
+        log("non-finite field after the run")
+        return 1
+
+    sites = float(n ** 3)
+    bps = 16.0 if dtype == np.float32 else 32.0
+    value = sites * a.steps / wall
+    kern_s = ev_ms * 1e-3 / a.steps
+    achieved = sites * bps / kern_s / 1e9
+    prof_name = "rk2_fused_f32_256cube.md" if dtype == np.float32 else "rk2_fused_f64_256cube.md"
+    traffic = traffic_from_profiles(prof_name)
+    out = {
+        "metric": "site-updates/s + HBM GB/s at 256^3 FP32; batched ensemble vs plain loop",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REF_CPU_SITE_UPDATES,
+        "dtype": "f32" if dtype == np.float32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"real scalar 256^3 periodic, {np.dtype(dtype).name}, fused RK2 "
+                               f"+ inline hash noise, 8192-pt V' table at T0=7000",
+                   "lattice": [n, n, n], "dt": prm["dt"], "dx": prm["dx"],
+                   "eta": prm["eta_eff"], "table_points": len(table)},
+        "hbm_gbps_algorithmic": sites * bps * a.steps / wall / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "kernel_us": kern_s * 1e6, "bytes_per_site": bps},
+        "true_vacuum_fraction": frac_true,
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)
+    lat.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -198,6 +198,45 @@ int tip_minmax_f64(const double *phi, int64_t n, double *out2, void *stream);
 int tip_rho_max_c32(const float *phi1, const float *phi2, int64_t n, double *out1,
                     void *stream);
 
+/* ---- device-resident lattice (the performance path) -----------------------------------
+ * Owns ping-pong field buffers in HBM, the V' table (float64 + float32 segments), a stream and
+ * HIP events, so state stays on the device between save points and each step is one kernel
+ * launch with no copy.  Replaces the reference's main loop body around rk2_step_table
+ * (simulation/latticeSimeRescale_numba.py:3619-3768; complex latticeSimComplex_numba.py:
+ * 4237-4398) for a fixed parameter set over nsteps; the host calls it once per schedule
+ * update (T / Hubble, latticeSimeRescale_numba.py:3622-3647).
+ * field_kind: TIP_REAL (phi, pi) or TIP_COMPLEX (phi1, phi2, pi1, pi2); precision 4 or 8
+ * (complex: 4 only).  Field index `which`: real 0 = phi, 1 = pi; complex 0 = phi1, 1 = pi1,
+ * 2 = phi2, 3 = pi2.  Step n of a call uses noise_seed = tip_noise_seed(step0 + n, seed). */
+#define TIP_REAL 0
+#define TIP_COMPLEX 1
+
+typedef struct tip_step_params {
+    double dt, dx, eta_eff, mu, inv_a2;
+    double noise_scale; /* 0 disables the inline hash noise */
+} tip_step_params;
+
+typedef struct tip_lattice tip_lattice;
+
+int tip_lattice_create(tip_lattice **out, int field_kind, int precision, int64_t nx, int64_t ny,
+                       int64_t nz);
+int tip_lattice_destroy(tip_lattice *lat);
+int tip_lattice_stream(tip_lattice *lat, void **stream);
+int tip_lattice_field_ptr(tip_lattice *lat, int which, void **ptr);
+int tip_lattice_upload(tip_lattice *lat, int which, const void *host);
+int tip_lattice_download(tip_lattice *lat, int which, void *host);
+int tip_lattice_set_table(tip_lattice *lat, const double *host_table, int64_t npts,
+                          double tmin, double dinv, int odd);
+int tip_lattice_set_zn(tip_lattice *lat, double zn_n, double zn_dv, double zn_active);
+/* nsteps RK2 steps; elapsed_ms (nullable) receives the HIP-event time of the launches on the
+ * lattice's stream and makes the call synchronous. */
+int tip_lattice_step(tip_lattice *lat, const tip_step_params *p, int64_t nsteps, uint64_t step0,
+                     uint64_t seed, float *elapsed_ms);
+/* Synchronous observables on the current state. */
+int tip_lattice_vacuum_count(tip_lattice *lat, double esc, unsigned long long *count);
+int tip_lattice_winding(tip_lattice *lat, int8_t *plaq_dev, unsigned long long *counts2);
+int tip_lattice_minmax(tip_lattice *lat, double *out2);
+
 #if defined(TIP_BUILDING) && defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

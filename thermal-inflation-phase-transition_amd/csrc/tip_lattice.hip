@@ -1,0 +1,273 @@
+// tip_lattice.hip -- device-resident lattice: ping-pong field buffers, the V' table in both
+// precisions, a stream and timing events.  One RK2 step = one fused kernel launch; the
+// buffers swap roles after every step, so no copy ever runs.
+#include "tip_internal.h"
+
+struct tip_lattice {
+    int kind = TIP_REAL;
+    int prec = 4;
+    int64_t nx = 0, ny = 0, nz = 0;
+    int nfields = 2;
+    void* buf[2][4] = {};  // [parity][field]
+    int cur = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* table = nullptr;
+    float2* table_ab = nullptr;
+    int64_t npts = 0, cap = 0;
+    double tmin = 0.0, dinv = 1.0;
+    int odd = 0;
+    double zn_n = 0.0, zn_dv = 0.0, zn_active = 0.0;
+    unsigned long long* counter = nullptr;  // 2 entries
+    double* mm = nullptr;                   // 2 entries
+};
+
+using namespace tip;
+
+namespace {
+
+size_t field_bytes(const tip_lattice* L) { return (size_t)(L->nx * L->ny * L->nz) * L->prec; }
+
+int release(tip_lattice* L)
+{
+    if (!L) return TIP_OK;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    for (auto& p : L->buf)
+        for (void*& q : p)
+            if (q) { (void)hipFree(q); q = nullptr; }
+    if (L->table) (void)hipFree(L->table);
+    if (L->table_ab) (void)hipFree(L->table_ab);
+    if (L->counter) (void)hipFree(L->counter);
+    if (L->mm) (void)hipFree(L->mm);
+    if (L->ev0) (void)hipEventDestroy(L->ev0);
+    if (L->ev1) (void)hipEventDestroy(L->ev1);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    delete L;
+    return TIP_OK;
+}
+
+// Map the public field index to the internal slot: real {phi, pi}; complex
+// {phi1, pi1, phi2, pi2} -> slots {0, 2, 1, 3} so that the kernel order is phi1, phi2, pi1, pi2.
+int slot_of(const tip_lattice* L, int which)
+{
+    if (which < 0 || which >= L->nfields) return -1;
+    if (L->kind == TIP_REAL) return which;
+    static const int m[4] = {0, 2, 1, 3};
+    return m[which];
+}
+
+}  // namespace
+
+extern "C" int tip_lattice_create(tip_lattice** out, int field_kind, int precision, int64_t nx,
+                                  int64_t ny, int64_t nz)
+{
+    TIP_REQUIRE(out, "tip_lattice_create: NULL out");
+    *out = nullptr;
+    TIP_REQUIRE(field_kind == TIP_REAL || field_kind == TIP_COMPLEX, "unknown field_kind %d",
+                field_kind);
+    TIP_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    TIP_REQUIRE(!(field_kind == TIP_COMPLEX && precision == 8),
+                "complex lattices are float32 (use tip_rk2_step_table_c64 for float64)");
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    tip_lattice* L = new tip_lattice();
+    L->kind = field_kind;
+    L->prec = precision;
+    L->nx = nx;
+    L->ny = ny;
+    L->nz = nz;
+    L->nfields = field_kind == TIP_REAL ? 2 : 4;
+    hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&L->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&L->ev1);
+    for (int p = 0; p < 2 && e == hipSuccess; ++p)
+        for (int f = 0; f < L->nfields && e == hipSuccess; ++f) {
+            e = hipMalloc(&L->buf[p][f], field_bytes(L));
+            if (e == hipSuccess) e = hipMemsetAsync(L->buf[p][f], 0, field_bytes(L), L->stream);
+        }
+    if (e == hipSuccess) e = hipMalloc(&L->counter, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&L->mm, 2 * sizeof(double));
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    if (e != hipSuccess) {
+        int r = fail_hip(e, "tip_lattice_create");
+        release(L);
+        return r;
+    }
+    *out = L;
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_destroy(tip_lattice* lat) { return release(lat); }
+
+extern "C" int tip_lattice_stream(tip_lattice* lat, void** stream)
+{
+    TIP_REQUIRE(lat && stream, "tip_lattice_stream: NULL");
+    *stream = lat->stream;
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_field_ptr(tip_lattice* lat, int which, void** ptr)
+{
+    TIP_REQUIRE(lat && ptr, "tip_lattice_field_ptr: NULL");
+    int s = slot_of(lat, which);
+    TIP_REQUIRE(s >= 0, "tip_lattice_field_ptr: bad field index %d", which);
+    *ptr = lat->buf[lat->cur][s];
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_upload(tip_lattice* lat, int which, const void* host)
+{
+    TIP_REQUIRE(lat && host, "tip_lattice_upload: NULL");
+    int s = slot_of(lat, which);
+    TIP_REQUIRE(s >= 0, "tip_lattice_upload: bad field index %d", which);
+    TIP_HIP(hipMemcpyAsync(lat->buf[lat->cur][s], host, field_bytes(lat), hipMemcpyHostToDevice,
+                           lat->stream));
+    TIP_HIP(hipStreamSynchronize(lat->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_download(tip_lattice* lat, int which, void* host)
+{
+    TIP_REQUIRE(lat && host, "tip_lattice_download: NULL");
+    int s = slot_of(lat, which);
+    TIP_REQUIRE(s >= 0, "tip_lattice_download: bad field index %d", which);
+    TIP_HIP(hipMemcpyAsync(host, lat->buf[lat->cur][s], field_bytes(lat), hipMemcpyDeviceToHost,
+                           lat->stream));
+    TIP_HIP(hipStreamSynchronize(lat->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_set_table(tip_lattice* lat, const double* host_table, int64_t npts,
+                                     double tmin, double dinv, int odd)
+{
+    TIP_REQUIRE(lat && host_table, "tip_lattice_set_table: NULL");
+    TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "tip_lattice_set_table: npts out of range");
+    TIP_REQUIRE(dinv > 0.0, "tip_lattice_set_table: dinv must be positive");
+    TIP_REQUIRE(!(lat->kind == TIP_COMPLEX && odd), "complex tables are tabulated on rho >= 0");
+    if (npts > lat->cap) {
+        TIP_HIP(hipStreamSynchronize(lat->stream));
+        if (lat->table) TIP_HIP(hipFree(lat->table));
+        if (lat->table_ab) TIP_HIP(hipFree(lat->table_ab));
+        lat->table = nullptr;
+        lat->table_ab = nullptr;
+        TIP_HIP(hipMalloc(&lat->table, npts * sizeof(double)));
+        TIP_HIP(hipMalloc(&lat->table_ab, (npts - 1) * sizeof(float2)));
+        lat->cap = npts;
+    }
+    TIP_HIP(hipMemcpyAsync(lat->table, host_table, npts * sizeof(double), hipMemcpyHostToDevice,
+                           lat->stream));
+    int rc = prepare_table_f32(lat->table, npts, tmin, dinv, lat->table_ab, lat->stream);
+    if (rc) return rc;
+    TIP_HIP(hipStreamSynchronize(lat->stream));
+    lat->npts = npts;
+    lat->tmin = tmin;
+    lat->dinv = dinv;
+    lat->odd = odd ? 1 : 0;
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_set_zn(tip_lattice* lat, double zn_n, double zn_dv, double zn_active)
+{
+    TIP_REQUIRE(lat, "tip_lattice_set_zn: NULL");
+    lat->zn_n = zn_n;
+    lat->zn_dv = zn_dv;
+    lat->zn_active = zn_active;
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_t nsteps,
+                                uint64_t step0, uint64_t seed, float* elapsed_ms)
+{
+    TIP_REQUIRE(L && p, "tip_lattice_step: NULL");
+    TIP_REQUIRE(L->npts >= 2, "tip_lattice_step: no V' table set");
+    TIP_REQUIRE(nsteps >= 0, "tip_lattice_step: nsteps < 0");
+    if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const uint64_t ns = tip_noise_seed(step0 + (uint64_t)n, seed);
+        void** in = L->buf[L->cur];
+        void** out = L->buf[L->cur ^ 1];
+        int rc;
+        if (L->kind == TIP_REAL && L->prec == 4) {
+            rc = tip_rk2_fused_f32((const float*)in[0], (const float*)in[1], (float*)out[0],
+                                   (float*)out[1], p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2,
+                                   nullptr, L->table_ab, L->tmin, L->dinv, L->npts, L->odd,
+                                   L->nx, L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0,
+                                   L->stream);
+        } else if (L->kind == TIP_REAL) {
+            rc = tip_rk2_fused_f64((const double*)in[0], (const double*)in[1], (double*)out[0],
+                                   (double*)out[1], p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2,
+                                   nullptr, L->table, L->tmin, L->dinv, L->npts, L->odd, L->nx,
+                                   L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0, L->stream);
+        } else {
+            rc = tip_rk2_fused_c32((const float*)in[0], (const float*)in[1], (const float*)in[2],
+                                   (const float*)in[3], (float*)out[0], (float*)out[1],
+                                   (float*)out[2], (float*)out[3], p->dt, p->dx, p->eta_eff,
+                                   p->mu, p->inv_a2, nullptr, nullptr, L->table_ab, L->tmin,
+                                   L->dinv, L->npts, L->zn_n, L->zn_dv, L->zn_active, L->nx,
+                                   L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0, L->stream);
+        }
+        if (rc) return rc;
+        L->cur ^= 1;
+    }
+    if (elapsed_ms) {
+        TIP_HIP(hipEventRecord(L->ev1, L->stream));
+        TIP_HIP(hipEventSynchronize(L->ev1));
+        TIP_HIP(hipEventElapsedTime(elapsed_ms, L->ev0, L->ev1));
+    }
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_vacuum_count(tip_lattice* L, double esc, unsigned long long* count)
+{
+    TIP_REQUIRE(L && count, "tip_lattice_vacuum_count: NULL");
+    TIP_HIP(hipMemsetAsync(L->counter, 0, sizeof(unsigned long long), L->stream));
+    int64_t n = L->nx * L->ny * L->nz;
+    void** b = L->buf[L->cur];
+    int rc;
+    if (L->kind == TIP_COMPLEX)
+        rc = tip_vacuum_count_c32((const float*)b[0], (const float*)b[1], n, esc, L->counter,
+                                  L->stream);
+    else if (L->prec == 4)
+        rc = tip_vacuum_count_f32((const float*)b[0], n, esc, L->counter, L->stream);
+    else
+        rc = tip_vacuum_count_f64((const double*)b[0], n, esc, L->counter, L->stream);
+    if (rc) return rc;
+    TIP_HIP(hipMemcpyAsync(count, L->counter, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           L->stream));
+    TIP_HIP(hipStreamSynchronize(L->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_winding(tip_lattice* L, int8_t* plaq_dev, unsigned long long* counts2)
+{
+    TIP_REQUIRE(L && counts2, "tip_lattice_winding: NULL");
+    TIP_REQUIRE(L->kind == TIP_COMPLEX, "tip_lattice_winding: needs a complex lattice");
+    TIP_HIP(hipMemsetAsync(L->counter, 0, 2 * sizeof(unsigned long long), L->stream));
+    void** b = L->buf[L->cur];
+    int rc = tip_winding_f32((const float*)b[0], (const float*)b[1], plaq_dev, nullptr,
+                             L->counter, L->nx, L->ny, L->nz, L->stream);
+    if (rc) return rc;
+    TIP_HIP(hipMemcpyAsync(counts2, L->counter, 2 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, L->stream));
+    TIP_HIP(hipStreamSynchronize(L->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_minmax(tip_lattice* L, double* out2)
+{
+    TIP_REQUIRE(L && out2, "tip_lattice_minmax: NULL");
+    int64_t n = L->nx * L->ny * L->nz;
+    void** b = L->buf[L->cur];
+    int rc;
+    if (L->kind == TIP_COMPLEX) {
+        TIP_HIP(hipMemsetAsync(L->mm, 0, 2 * sizeof(double), L->stream));
+        rc = tip_rho_max_c32((const float*)b[0], (const float*)b[1], n, L->mm + 1, L->stream);
+    } else if (L->prec == 4)
+        rc = tip_minmax_f32((const float*)b[0], n, L->mm, L->stream);
+    else
+        rc = tip_minmax_f64((const double*)b[0], n, L->mm, L->stream);
+    if (rc) return rc;
+    TIP_HIP(hipMemcpyAsync(out2, L->mm, 2 * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    TIP_HIP(hipStreamSynchronize(L->stream));
+    return TIP_OK;
+}

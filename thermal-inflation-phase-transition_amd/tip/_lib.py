@@ -19,6 +19,15 @@ _u64 = C.c_uint64
 _int = C.c_int
 _sz = C.c_size_t
 _pint = C.POINTER(C.c_int)
+
+
+class StepParams(C.Structure):
+    """tip_step_params of include/tip.h."""
+
+    _fields_ = [("dt", C.c_double), ("dx", C.c_double), ("eta_eff", C.c_double),
+                ("mu", C.c_double), ("inv_a2", C.c_double), ("noise_scale", C.c_double)]
+
+
 _ppv = C.POINTER(C.c_void_p)
 
 # name -> (restype, argtypes); every symbol of include/tip.h
@@ -71,6 +80,19 @@ SIGNATURES = {
     "tip_minmax_f32": (_int, [_vp, _i64, _vp, _vp]),
     "tip_minmax_f64": (_int, [_vp, _i64, _vp, _vp]),
     "tip_rho_max_c32": (_int, [_vp, _vp, _i64, _vp, _vp]),
+    "tip_lattice_create": (_int, [_ppv, _int, _int, _i64, _i64, _i64]),
+    "tip_lattice_destroy": (_int, [_vp]),
+    "tip_lattice_stream": (_int, [_vp, _ppv]),
+    "tip_lattice_field_ptr": (_int, [_vp, _int, _ppv]),
+    "tip_lattice_upload": (_int, [_vp, _int, _vp]),
+    "tip_lattice_download": (_int, [_vp, _int, _vp]),
+    "tip_lattice_set_table": (_int, [_vp, _vp, _i64, _d, _d, _int]),
+    "tip_lattice_set_zn": (_int, [_vp, _d, _d, _d]),
+    "tip_lattice_step": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
+                                C.POINTER(C.c_float)]),
+    "tip_lattice_vacuum_count": (_int, [_vp, _d, C.POINTER(C.c_ulonglong)]),
+    "tip_lattice_winding": (_int, [_vp, _vp, C.POINTER(C.c_ulonglong)]),
+    "tip_lattice_minmax": (_int, [_vp, C.POINTER(C.c_double)]),
 }
 
 

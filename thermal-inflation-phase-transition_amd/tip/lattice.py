@@ -1,0 +1,110 @@
+"""Device-resident lattice (the performance path of the hot loop).
+
+A Lattice owns ping-pong (phi, pi) buffers in HBM (complex: phi1, phi2, pi1, pi2), the V'
+table in float64 and float32-segment form, and a HIP stream.  step(n) runs n fused RK2
+kernels back to back with the reference's hash noise (noise_seed = step ^ seed*golden, the
+reference's uint64(step) for seed 0, simulation/latticeSimeRescale_numba.py:3738), with no
+host round trip; observables (true-vacuum count, winding, min/max) run on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import StepParams
+
+REAL, COMPLEX = 0, 1
+
+
+class Lattice:
+    def __init__(self, shape, dtype=np.float32, complex_field: bool = False):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.complex = bool(complex_field)
+        h = C.c_void_p()
+        _lib.call("tip_lattice_create", C.byref(h), COMPLEX if self.complex else REAL,
+                  self.dtype.itemsize, *self.shape)
+        self._h = h
+        self.step_count = 0
+        self.table_info = None
+
+    # -- state ---------------------------------------------------------------------------
+    @property
+    def handle(self):
+        return self._h
+
+    def _idx(self, name: str) -> int:
+        names = ["phi1", "pi1", "phi2", "pi2"] if self.complex else ["phi", "pi"]
+        return names.index(name)
+
+    def upload(self, name: str, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        if x.shape != self.shape:
+            raise ValueError(f"{name}: shape {x.shape} != {self.shape}")
+        _lib.call("tip_lattice_upload", self._h, self._idx(name), x.ctypes.data)
+
+    def download(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _lib.call("tip_lattice_download", self._h, self._idx(name), out.ctypes.data)
+        return out
+
+    def field_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        _lib.call("tip_lattice_field_ptr", self._h, self._idx(name), C.byref(p))
+        return int(p.value)
+
+    def set_table(self, table: np.ndarray, tmin: float, dinv: float, odd: bool = False) -> None:
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        _lib.call("tip_lattice_set_table", self._h, t.ctypes.data, t.size, float(tmin),
+                  float(dinv), int(bool(odd)))
+        self.table_info = (float(t[0]), float(tmin), float(dinv), t.size, bool(odd))
+
+    def set_zn(self, n: float, dv: float, active: float) -> None:
+        _lib.call("tip_lattice_set_zn", self._h, float(n), float(dv), float(active))
+
+    # -- evolution -----------------------------------------------------------------------
+    def step(self, nsteps: int, *, dt: float, dx: float, eta_eff: float, mu: float,
+             inv_a2: float = 1.0, noise_scale: float = 0.0, seed: int = 0,
+             step0: Optional[int] = None, timed: bool = False) -> Optional[float]:
+        """Advance nsteps; returns the HIP-event milliseconds of the launches if timed."""
+        p = StepParams(dt, dx, eta_eff, mu, inv_a2, noise_scale)
+        s0 = self.step_count if step0 is None else int(step0)
+        ms = C.c_float(0.0)
+        _lib.call("tip_lattice_step", self._h, C.byref(p), int(nsteps), s0 & 0xFFFFFFFFFFFFFFFF,
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(ms) if timed else None)
+        self.step_count = s0 + int(nsteps)
+        return float(ms.value) if timed else None
+
+    # -- observables ---------------------------------------------------------------------
+    def vacuum_count(self, esc: float = 1e4) -> int:
+        c = C.c_ulonglong(0)
+        _lib.call("tip_lattice_vacuum_count", self._h, float(esc), C.byref(c))
+        return int(c.value)
+
+    def vacuum_fraction(self, esc: float = 1e4) -> float:
+        return self.vacuum_count(esc) / float(np.prod(self.shape))
+
+    def winding_counts(self, plaq_ptr: Optional[int] = None):
+        """(string sites, nonzero plaquettes) of the current complex field."""
+        c = (C.c_ulonglong * 2)()
+        _lib.call("tip_lattice_winding", self._h, plaq_ptr, c)
+        return int(c[0]), int(c[1])
+
+    def minmax(self):
+        out = (C.c_double * 2)()
+        _lib.call("tip_lattice_minmax", self._h, out)
+        return float(out[0]), float(out[1])
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().tip_lattice_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
