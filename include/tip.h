@@ -228,6 +228,11 @@ int tip_lattice_download(tip_lattice *lat, int which, void *host);
 int tip_lattice_set_table(tip_lattice *lat, const double *host_table, int64_t npts,
                           double tmin, double dinv, int odd);
 int tip_lattice_set_zn(tip_lattice *lat, double zn_n, double zn_dv, double zn_active);
+/* Launch geometry of the float32 step kernels (the autotuner's knobs): rows of 64 z-sites per
+ * block (1, 2, 4, 8; 0 = default), x planes each block marches through (0 = derived), and the
+ * XCD-aware block remap (0/1). */
+int tip_lattice_set_geometry(tip_lattice *lat, int rows_per_block, int planes_per_block,
+                             int xcd_remap);
 /* nsteps RK2 steps; elapsed_ms (nullable) receives the HIP-event time of the launches on the
  * lattice's stream and makes the call synchronous. */
 int tip_lattice_step(tip_lattice *lat, const tip_step_params *p, int64_t nsteps, uint64_t step0,

@@ -283,3 +283,85 @@ def test_vacuum_counts_exact():
     a = (rng.normal(size=(40, 50, 60)) * 1e4).astype(np.float32)
     b = (rng.normal(size=(40, 50, 60)) * 1e4).astype(np.float32)
     assert K.vacuum_count(a, 1e4, b) == O.vacuum_count(a, 1e4, b)
+
+
+# ----------------------------------------------------------------------------------------
+# device-resident lattice: geometry independence and full-size properties
+# ----------------------------------------------------------------------------------------
+def test_lattice_geometries_bitwise_identical():
+    """Every launch geometry computes the same per-site arithmetic: results are identical."""
+    from tip.lattice import Lattice
+    rng = np.random.default_rng(8)
+    shape = (40, 36, 96)
+    phi0 = (rng.normal(size=shape) * 1e3).astype(np.float32)
+    pi0 = (rng.normal(size=shape) * 1e2).astype(np.float32)
+    t, tmin, dinv = c0_table(1024)
+    prm = c0_params()
+    kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+              noise_scale=prm["noise_scale"])
+    ref = None
+    for by, cx, xcd in [(0, 0, 1), (1, 3, 0), (2, 7, 1), (4, 40, 0), (8, 1, 1), (8, 16, 0)]:
+        lat = Lattice(shape, np.float32)
+        lat.upload("phi", phi0)
+        lat.upload("pi", pi0)
+        lat.set_table(t, tmin, dinv, odd=True)
+        lat.set_geometry(by, cx, xcd)
+        lat.step(5, seed=3, **kw)
+        out = (lat.download("phi"), lat.download("pi"))
+        lat.close()
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (by, cx, xcd)
+    # and the drop-in entry point gives the same numbers
+    from tip import kernels as K
+    g, h = phi0.copy(), pi0.copy()
+    for n in range(5):
+        K.rk2_step_table(g, h, prm["dt"], prm["dx"], prm["eta_eff"], prm["mu"], None,
+                         np.empty_like(g), np.empty_like(h), 1.0, t, tmin, dinv, len(t), odd=True,
+                         noise_seed=O.noise_seed(n, 3), noise_scale=prm["noise_scale"])
+    assert np.array_equal(g, ref[0]) and np.array_equal(h, ref[1])
+
+
+def test_full_size_256_uniform_mode_closed_form():
+    """256^3 float32, no noise, linear table: a uniform field follows the exact RK2 matrix."""
+    from tip.lattice import Lattice
+    n = 256
+    c = -1.0e6
+    x = np.linspace(-1e6, 1e6, 8193)
+    t, tmin, dinv = c * x, -1e6, 8192 / 2e6
+    lat = Lattice((n, n, n), np.float32)
+    lat.upload("phi", np.full((n, n, n), 300.0, np.float32))
+    lat.upload("pi", np.full((n, n, n), -50.0, np.float32))
+    lat.set_table(t, tmin, dinv)
+    dt, eta, mu = 0.25, 7.0, 1000.0
+    lat.step(100, dt=dt, dx=2.0, eta_eff=eta, mu=mu, noise_scale=0.0)
+    A = np.array([[0.0, 1.0], [-c / mu ** 2, -eta]])
+    M = np.eye(2) + dt * A + 0.5 * dt * dt * A @ A
+    y = np.linalg.matrix_power(M, 100) @ np.array([300.0, -50.0])
+    phi, pi = lat.download("phi"), lat.download("pi")
+    assert np.all(phi == phi[0, 0, 0]) and np.all(pi == pi[0, 0, 0])  # stays uniform
+    assert abs(phi[0, 0, 0] - y[0]) <= 1e-4 * abs(y[0])
+    assert abs(pi[0, 0, 0] - y[1]) <= 1e-4 * abs(y[1])
+    lo, hi = lat.minmax()
+    assert lo == hi == float(phi[0, 0, 0])
+    lat.close()
+
+
+def test_full_size_256_noise_statistics():
+    """256^3: one step from rest with no force isolates the noise: mean 0, var ns^2."""
+    from tip.lattice import Lattice
+    n = 256
+    lat = Lattice((n, n, n), np.float32)
+    lat.set_table(np.zeros(2), 0.0, 1.0)
+    ns = 1750.0
+    lat.step(1, dt=0.25, dx=2.0, eta_eff=7.0, mu=1000.0, noise_scale=ns, step0=42)
+    pi = lat.download("pi").astype(np.float64).ravel()
+    assert np.all(lat.download("phi") == 0.0)
+    m, v = pi.mean(), pi.var()
+    assert abs(m) < 5 * ns / math.sqrt(pi.size)
+    assert abs(v / ns ** 2 - 1.0) < 5 * math.sqrt(2.0 / pi.size)
+    # the same sites of the oracle stream (first 4096 sites), float32 transcendentals
+    z = O.noise_field((1, 16, 256), 42, ns)
+    np.testing.assert_allclose(pi[:4096], z.ravel(), rtol=0, atol=2e-6 * ns * 8)
+    lat.close()

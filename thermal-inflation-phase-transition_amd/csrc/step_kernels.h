@@ -62,17 +62,18 @@ __device__ __forceinline__ unsigned wave_count(bool p)
 // ---------------------------------------------------------------------------------------
 // Real field.
 // ---------------------------------------------------------------------------------------
+// One (z, y) tile of 64 x BY columns marching over planes [x0, x1).  Returns the wave's
+// count of sites with |phi_new| > esc when `count` is set (all 64 lanes of every wave must
+// call it with the same x0, x1 and count).
 template <typename R, typename Tab, int BY>
-__global__ __launch_bounds__(64 * BY) void k_rk2_real(
+__device__ __forceinline__ unsigned rk2_real_tile(
     const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
-    R* __restrict__ pi_out, const R* __restrict__ noise, StepScalars<R> c, Tab tab, int nx,
-    int ny, int nz, int cx, uint64_t noise_seed, int hash_noise,
-    unsigned long long* __restrict__ vac, R esc)
+    R* __restrict__ pi_out, const R* __restrict__ noise, const StepScalars<R>& c,
+    const Tab& tab, int nx, int ny, int nz, int tile_z, int tile_y, int x0, int x1,
+    uint64_t noise_seed, int hash_noise, bool count, R esc)
 {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int x0 = blockIdx.z * cx;
-    const int x1 = min(x0 + cx, nx);
+    const int k = tile_z * 64 + (int)threadIdx.x;
+    const int j = tile_y * BY + (int)threadIdx.y;
     const bool act = (k < nz) && (j < ny);
     const int kk = act ? k : 0, jj = act ? j : 0;
     const int nyz = ny * nz;
@@ -146,9 +147,43 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_real(
             pi_out[s] = qn;
             tv = r_abs(pn) > esc;
         }
-        if (vac) cnt += wave_count(tv);
+        if (count) cnt += wave_count(tv);
         pm = pc; qm = qc; pc = pp; qc = qp;
     }
+    return cnt;
+}
+
+// Block -> tile decode.  Blocks are numbered with the z tile fastest, then the y tile, then
+// the x chunk.  With `xcd_remap` the hardware's round-robin dispatch of consecutive block ids
+// over the 8 XCDs is undone: block b runs tile (b % 8) * (nb / 8) + b / 8, so each XCD works
+// through one contiguous range of x chunks and the neighbour planes a chunk re-reads stay in
+// that XCD's L2.  Requires nb % 8 == 0 (the launcher clears the flag otherwise).
+struct TileGrid {
+    int tiles_z, tiles_y, chunks, cx, xcd_remap;
+    __device__ __forceinline__ void decode(int b, int nb, int& tz, int& ty, int& ch) const
+    {
+        if (xcd_remap) b = (b & 7) * (nb >> 3) + (b >> 3);
+        tz = b % tiles_z;
+        int r = b / tiles_z;
+        ty = r % tiles_y;
+        ch = r / tiles_y;
+    }
+};
+
+template <typename R, typename Tab, int BY>
+__global__ __launch_bounds__(64 * BY) void k_rk2_real(
+    const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
+    R* __restrict__ pi_out, const R* __restrict__ noise, StepScalars<R> c, Tab tab, int nx,
+    int ny, int nz, TileGrid g, uint64_t noise_seed, int hash_noise,
+    unsigned long long* __restrict__ vac, R esc)
+{
+    int tz, ty, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tz, ty, ch);
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
+    unsigned cnt = rk2_real_tile<R, Tab, BY>(phi, pi, phi_out, pi_out, noise, c, tab, nx, ny, nz,
+                                             tz, ty, x0, x1, noise_seed, hash_noise,
+                                             vac != nullptr, esc);
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
@@ -210,13 +245,15 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
     const R* __restrict__ pi2, R* __restrict__ phi1_out, R* __restrict__ phi2_out,
     R* __restrict__ pi1_out, R* __restrict__ pi2_out, const R* __restrict__ noise1,
     const R* __restrict__ noise2, StepScalars<R> c, Tab tab, ZnParams<R> zn, int nx, int ny,
-    int nz, int cx, uint64_t noise_seed, int hash_noise, unsigned long long* __restrict__ vac,
-    R esc2)
+    int nz, TileGrid g, uint64_t noise_seed, int hash_noise,
+    unsigned long long* __restrict__ vac, R esc2)
 {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int x0 = blockIdx.z * cx;
-    const int x1 = min(x0 + cx, nx);
+    int tz, ty, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tz, ty, ch);
+    const int k = tz * 64 + (int)threadIdx.x;
+    const int j = ty * BY + (int)threadIdx.y;
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
     const bool act = (k < nz) && (j < ny);
     const int kk = act ? k : 0, jj = act ? j : 0;
     const int nyz = ny * nz;
@@ -323,9 +360,28 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
 
 // Launch geometry shared by the real and complex launchers.
 struct StepGeometry {
-    int by;  // rows per block (1, 2, 4 or 8)
-    int cx;  // x planes per block
+    int by;   // rows per block (1, 2, 4 or 8)
+    int cx;   // x planes per block (0 = choose from the lattice size)
+    int xcd;  // 1 = XCD-aware block remap
 };
 StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field, int f64);
+// Resolve a requested geometry (nullable = default) into the tile grid and the block count.
+int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz,
+                     int complex_field, int f64, StepGeometry& g, TileGrid& tg, int& nblocks);
+
+// Launchers shared by the C-ABI entry points, the lattice and the ensemble.
+int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out,
+                    const StepScalars<float>& c, const float* noise, const TableF32& tab,
+                    int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
+                    unsigned long long* vac, float esc, const StepGeometry* geom,
+                    hipStream_t st);
+int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
+                       float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
+                       const float* noise1, const float* noise2, const TableF32& tab,
+                       const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
+                       uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
+                       const StepGeometry* geom, hipStream_t st);
+TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd);
+ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active);
 
 }  // namespace tip

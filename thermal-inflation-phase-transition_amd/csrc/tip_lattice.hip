@@ -1,7 +1,7 @@
 // tip_lattice.hip -- device-resident lattice: ping-pong field buffers, the V' table in both
 // precisions, a stream and timing events.  One RK2 step = one fused kernel launch; the
 // buffers swap roles after every step, so no copy ever runs.
-#include "tip_internal.h"
+#include "step_kernels.h"
 
 struct tip_lattice {
     int kind = TIP_REAL;
@@ -20,6 +20,7 @@ struct tip_lattice {
     double zn_n = 0.0, zn_dv = 0.0, zn_active = 0.0;
     unsigned long long* counter = nullptr;  // 2 entries
     double* mm = nullptr;                   // 2 entries
+    tip::StepGeometry geom = {0, 0, 1};     // by = 0: the default for the lattice size
 };
 
 using namespace tip;
@@ -175,12 +176,32 @@ extern "C" int tip_lattice_set_zn(tip_lattice* lat, double zn_n, double zn_dv, d
     return TIP_OK;
 }
 
+extern "C" int tip_lattice_set_geometry(tip_lattice* lat, int rows_per_block,
+                                        int planes_per_block, int xcd_remap)
+{
+    TIP_REQUIRE(lat, "tip_lattice_set_geometry: NULL");
+    StepGeometry req = {rows_per_block, planes_per_block, xcd_remap};
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    int rc = resolve_geometry(&req, lat->nx, lat->ny, lat->nz, lat->kind == TIP_COMPLEX, 0, g, tg,
+                              nb);
+    if (rc) return rc;
+    lat->geom = req;
+    return TIP_OK;
+}
+
 extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_t nsteps,
                                 uint64_t step0, uint64_t seed, float* elapsed_ms)
 {
     TIP_REQUIRE(L && p, "tip_lattice_step: NULL");
     TIP_REQUIRE(L->npts >= 2, "tip_lattice_step: no V' table set");
     TIP_REQUIRE(nsteps >= 0, "tip_lattice_step: nsteps < 0");
+    const StepScalars<float> c32 =
+        make_scalars<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale);
+    const TableF32 tab32 = make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd);
+    const ZnParams<float> zn32 = make_zn32(L->zn_n, L->zn_dv, L->zn_active);
+    const int hash = p->noise_scale != 0.0 ? 1 : 0;
     if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
     for (int64_t n = 0; n < nsteps; ++n) {
         const uint64_t ns = tip_noise_seed(step0 + (uint64_t)n, seed);
@@ -188,23 +209,20 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
         void** out = L->buf[L->cur ^ 1];
         int rc;
         if (L->kind == TIP_REAL && L->prec == 4) {
-            rc = tip_rk2_fused_f32((const float*)in[0], (const float*)in[1], (float*)out[0],
-                                   (float*)out[1], p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2,
-                                   nullptr, L->table_ab, L->tmin, L->dinv, L->npts, L->odd,
-                                   L->nx, L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0,
-                                   L->stream);
+            rc = launch_real_f32((const float*)in[0], (const float*)in[1], (float*)out[0],
+                                 (float*)out[1], c32, nullptr, tab32, L->nx, L->ny, L->nz, ns,
+                                 hash, nullptr, 0.0f, &L->geom, L->stream);
         } else if (L->kind == TIP_REAL) {
             rc = tip_rk2_fused_f64((const double*)in[0], (const double*)in[1], (double*)out[0],
                                    (double*)out[1], p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2,
                                    nullptr, L->table, L->tmin, L->dinv, L->npts, L->odd, L->nx,
                                    L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0, L->stream);
         } else {
-            rc = tip_rk2_fused_c32((const float*)in[0], (const float*)in[1], (const float*)in[2],
-                                   (const float*)in[3], (float*)out[0], (float*)out[1],
-                                   (float*)out[2], (float*)out[3], p->dt, p->dx, p->eta_eff,
-                                   p->mu, p->inv_a2, nullptr, nullptr, L->table_ab, L->tmin,
-                                   L->dinv, L->npts, L->zn_n, L->zn_dv, L->zn_active, L->nx,
-                                   L->ny, L->nz, ns, p->noise_scale, nullptr, 0.0, L->stream);
+            rc = launch_complex_f32((const float*)in[0], (const float*)in[1], (const float*)in[2],
+                                    (const float*)in[3], (float*)out[0], (float*)out[1],
+                                    (float*)out[2], (float*)out[3], c32, nullptr, nullptr, tab32,
+                                    zn32, L->nx, L->ny, L->nz, ns, hash, nullptr, 0.0f, &L->geom,
+                                    L->stream);
         }
         if (rc) return rc;
         L->cur ^= 1;

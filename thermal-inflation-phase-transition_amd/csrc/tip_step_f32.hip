@@ -1,5 +1,5 @@
-// tip_step_f32.hip -- float32 fused RK2 step (the performance path) and the float32 segment
-// table it reads.
+// tip_step_f32.hip -- float32 fused RK2 step (the performance path), its launch geometry and
+// the float32 segment table it reads.
 #include "step_kernels.h"
 
 using namespace tip;
@@ -39,6 +39,7 @@ StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field
     (void)f64;
     StepGeometry g;
     g.by = 4;
+    g.xcd = 1;
     int64_t per_plane = ((nz + 63) / 64) * ((ny + 3) / 4);
     // Aim for about 4096 blocks (16 per CU) while marching at least 1 and at most 32 planes.
     int64_t cx = (nx * per_plane) / 4096;
@@ -48,9 +49,33 @@ StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field
     return g;
 }
 
-}  // namespace tip
-
-namespace {
+int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz,
+                     int complex_field, int f64, StepGeometry& g, TileGrid& tg, int& nblocks)
+{
+    g = step_geometry(nx, ny, nz, complex_field, f64);
+    if (req) {
+        TIP_REQUIRE(req->by == 0 || req->by == 1 || req->by == 2 || req->by == 4 || req->by == 8,
+                    "geometry: rows per block must be 1, 2, 4 or 8 (got %d)", req->by);
+        TIP_REQUIRE(req->cx >= 0 && req->cx <= 4096, "geometry: planes per block out of range");
+        if (req->by) g.by = req->by;
+        if (req->by && !req->cx) {  // re-derive the plane count for the new row count
+            int64_t per_plane = ((nz + 63) / 64) * ((ny + g.by - 1) / g.by);
+            int64_t cx = (nx * per_plane) / 4096;
+            g.cx = (int)(cx < 1 ? 1 : (cx > 32 ? 32 : cx));
+        }
+        if (req->cx) g.cx = req->cx;
+        g.xcd = req->xcd ? 1 : 0;
+    }
+    tg.tiles_z = (int)((nz + 63) / 64);
+    tg.tiles_y = (int)((ny + g.by - 1) / g.by);
+    tg.chunks = (int)((nx + g.cx - 1) / g.cx);
+    tg.cx = g.cx;
+    int64_t nb = (int64_t)tg.tiles_z * tg.tiles_y * tg.chunks;
+    TIP_REQUIRE(nb < (int64_t(1) << 31), "geometry: too many blocks");
+    nblocks = (int)nb;
+    tg.xcd_remap = (g.xcd && nb % 8 == 0) ? 1 : 0;
+    return TIP_OK;
+}
 
 TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd)
 {
@@ -63,27 +88,86 @@ TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, in
     return tab;
 }
 
-int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out, double dt,
-                    double dx, double eta_eff, double mu, double inv_a2, const float* noise,
-                    const void* ab, double vp_tmin, double vp_dinv, int64_t vp_npts, int vp_odd,
-                    int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
-                    unsigned long long* vac, double vac_esc, hipStream_t st)
+ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active)
+{
+    ZnParams<float> zn;
+    zn.n = (float)zn_n;
+    zn.dv = (float)zn_dv;
+    zn.active = (zn_active > 0.5 && zn_n > 0.5) ? 1 : 0;
+    double r = zn_n - (double)(int64_t)zn_n;
+    zn.ipow = (r == 0.0 && zn_n >= 1.0 && zn_n <= 32.0) ? (int)zn_n : 0;
+    return zn;
+}
+
+int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out,
+                    const StepScalars<float>& c, const float* noise, const TableF32& tab,
+                    int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
+                    unsigned long long* vac, float esc, const StepGeometry* geom,
+                    hipStream_t st)
 {
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
-    TIP_REQUIRE(phi && pi && phi_out && pi_out && ab, "rk2 f32: NULL field or table");
-    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 f32: vp_npts out of range");
+    TIP_REQUIRE(phi && pi && phi_out && pi_out && tab.ab, "rk2 f32: NULL field or table");
     TIP_REQUIRE(phi != phi_out && pi != pi_out, "rk2 f32: output must not alias input");
-    StepScalars<float> c = make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale);
-    StepGeometry g = step_geometry(nx, ny, nz, 0, 0);
-    dim3 block(64, 4);
-    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
-              (unsigned)((nx + g.cx - 1) / g.cx));
-    int hash = (noise == nullptr && noise_scale != 0.0) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk2_real<float, TableF32, 4>), grid, block, 0, st, phi, pi, phi_out,
-                       pi_out, noise, c, make_table32(ab, vp_tmin, vp_dinv, vp_npts, vp_odd),
-                       (int)nx, (int)ny, (int)nz, g.cx, noise_seed, hash, vac, (float)vac_esc);
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    rc = resolve_geometry(geom, nx, ny, nz, 0, 0, g, tg, nb);
+    if (rc) return rc;
+#define TIP_LAUNCH_REAL(BYV)                                                                   \
+    hipLaunchKernelGGL((k_rk2_real<float, TableF32, BYV>), dim3(nb), dim3(64, BYV), 0, st, phi, \
+                       pi, phi_out, pi_out, noise, c, tab, (int)nx, (int)ny, (int)nz, tg,      \
+                       noise_seed, hash, vac, esc)
+    switch (g.by) {
+        case 1: TIP_LAUNCH_REAL(1); break;
+        case 2: TIP_LAUNCH_REAL(2); break;
+        case 8: TIP_LAUNCH_REAL(8); break;
+        default: TIP_LAUNCH_REAL(4); break;
+    }
+#undef TIP_LAUNCH_REAL
     TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
+                       float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
+                       const float* noise1, const float* noise2, const TableF32& tab,
+                       const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
+                       uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
+                       const StepGeometry* geom, hipStream_t st)
+{
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && tab.ab,
+                "rk2 c32: NULL field or table");
+    TIP_REQUIRE(noise1 != nullptr || noise2 == nullptr, "rk2 c32: noise2 without noise1");
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    rc = resolve_geometry(geom, nx, ny, nz, 1, 0, g, tg, nb);
+    if (rc) return rc;
+#define TIP_LAUNCH_CPLX(BYV)                                                                    \
+    hipLaunchKernelGGL((k_rk2_complex<float, TableF32, BYV>), dim3(nb), dim3(64, BYV), 0, st,    \
+                       phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, tab, zn, (int)nx, \
+                       (int)ny, (int)nz, tg, noise_seed, hash, vac, esc2)
+    switch (g.by) {
+        case 1: TIP_LAUNCH_CPLX(1); break;
+        case 2: TIP_LAUNCH_CPLX(2); break;
+        case 8: TIP_LAUNCH_CPLX(8); break;
+        default: TIP_LAUNCH_CPLX(4); break;
+    }
+#undef TIP_LAUNCH_CPLX
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+}  // namespace tip
+
+namespace {
+
+int check_npts(int64_t vp_npts)
+{
+    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 f32: vp_npts out of range");
     return TIP_OK;
 }
 
@@ -103,9 +187,13 @@ extern "C" int tip_rk2_fused_f32(const float* phi, const float* pi, float* phi_o
                                  int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
                                  unsigned long long* vac_count, double vac_esc, void* stream)
 {
-    return launch_real_f32(phi, pi, phi_out, pi_out, dt, dx, eta_eff, mu, inv_a2, noise, table_ab,
-                           vp_tmin, vp_dinv, vp_npts, vp_odd, nx, ny, nz, noise_seed, noise_scale,
-                           vac_count, vac_esc, as_stream(stream));
+    int rc = check_npts(vp_npts);
+    if (rc) return rc;
+    return launch_real_f32(phi, pi, phi_out, pi_out,
+                           make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), noise,
+                           make_table32(table_ab, vp_tmin, vp_dinv, vp_npts, vp_odd), nx, ny, nz,
+                           noise_seed, (noise == nullptr && noise_scale != 0.0) ? 1 : 0,
+                           vac_count, (float)vac_esc, nullptr, as_stream(stream));
 }
 
 extern "C" int tip_rk2_step_table_f32(float* phi, float* pi, double dt, double dx,
@@ -120,64 +208,23 @@ extern "C" int tip_rk2_step_table_f32(float* phi, float* pi, double dt, double d
     hipStream_t st = as_stream(stream);
     TIP_REQUIRE(phi_tmp && pi_tmp, "rk2_step_table_f32: phi_tmp/pi_tmp are required");
     TIP_REQUIRE(vp_table, "rk2_step_table_f32: NULL table");
-    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 f32: vp_npts out of range");
+    int rc = check_npts(vp_npts);
+    if (rc) return rc;
     float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
     TIP_REQUIRE(ab != nullptr, "rk2_step_table_f32: scratch allocation failed");
-    int rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
+    rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
     if (rc) return rc;
-    rc = launch_real_f32(phi, pi, phi_tmp, pi_tmp, dt, dx, eta_eff, mu, inv_a2, noise, ab, vp_tmin,
-                         vp_dinv, vp_npts, vp_odd, nx, ny, nz, noise_seed, noise_scale, vac_count,
-                         vac_esc, st);
+    rc = launch_real_f32(phi, pi, phi_tmp, pi_tmp,
+                         make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), noise,
+                         make_table32(ab, vp_tmin, vp_dinv, vp_npts, vp_odd), nx, ny, nz,
+                         noise_seed, (noise == nullptr && noise_scale != 0.0) ? 1 : 0, vac_count,
+                         (float)vac_esc, nullptr, st);
     if (rc) return rc;
     size_t bytes = (size_t)(nx * ny * nz) * sizeof(float);
     TIP_HIP(hipMemcpyAsync(phi, phi_tmp, bytes, hipMemcpyDeviceToDevice, st));
     TIP_HIP(hipMemcpyAsync(pi, pi_tmp, bytes, hipMemcpyDeviceToDevice, st));
     return TIP_OK;
 }
-
-namespace {
-
-ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active)
-{
-    ZnParams<float> zn;
-    zn.n = (float)zn_n;
-    zn.dv = (float)zn_dv;
-    zn.active = (zn_active > 0.5 && zn_n > 0.5) ? 1 : 0;
-    double r = zn_n - (double)(int64_t)zn_n;
-    zn.ipow = (r == 0.0 && zn_n >= 1.0 && zn_n <= 32.0) ? (int)zn_n : 0;
-    return zn;
-}
-
-int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
-                       float* o1, float* o2, float* o3, float* o4, double dt, double dx,
-                       double eta_eff, double mu, double inv_a2, const float* noise1,
-                       const float* noise2, const void* ab, double vp_tmin, double vp_dinv,
-                       int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
-                       int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
-                       unsigned long long* vac, double vac_esc, hipStream_t st)
-{
-    int rc = check_dims(nx, ny, nz);
-    if (rc) return rc;
-    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && ab,
-                "rk2 c32: NULL field or table");
-    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 c32: vp_npts out of range");
-    TIP_REQUIRE(noise1 != nullptr || noise2 == nullptr, "rk2 c32: noise2 without noise1");
-    StepScalars<float> c = make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale);
-    StepGeometry g = step_geometry(nx, ny, nz, 1, 0);
-    dim3 block(64, 4);
-    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
-              (unsigned)((nx + g.cx - 1) / g.cx));
-    int hash = (noise1 == nullptr && noise_scale != 0.0) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk2_complex<float, TableF32, 4>), grid, block, 0, st, phi1, phi2, pi1,
-                       pi2, o1, o2, o3, o4, noise1, noise2, c,
-                       make_table32(ab, vp_tmin, vp_dinv, vp_npts, 0),
-                       make_zn32(zn_n, zn_dv, zn_active), (int)nx, (int)ny, (int)nz, g.cx,
-                       noise_seed, hash, vac, (float)(vac_esc * vac_esc));
-    TIP_HIP(hipGetLastError());
-    return TIP_OK;
-}
-
-}  // namespace
 
 extern "C" int tip_rk2_fused_c32(const float* phi1, const float* phi2, const float* pi1,
                                  const float* pi2, float* phi1_out, float* phi2_out,
@@ -191,10 +238,14 @@ extern "C" int tip_rk2_fused_c32(const float* phi1, const float* phi2, const flo
 {
     TIP_REQUIRE(phi1 != phi1_out && phi2 != phi2_out && pi1 != pi1_out && pi2 != pi2_out,
                 "rk2 c32: output must not alias input");
-    return launch_complex_f32(phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out, pi2_out, dt, dx,
-                              eta_eff, mu, inv_a2, noise1, noise2, table_ab, vp_tmin, vp_dinv,
-                              vp_npts, zn_n, zn_dv, zn_active, nx, ny, nz, noise_seed,
-                              noise_scale, vac_count, vac_esc, as_stream(stream));
+    int rc = check_npts(vp_npts);
+    if (rc) return rc;
+    return launch_complex_f32(phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out, pi2_out,
+                              make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale),
+                              noise1, noise2, make_table32(table_ab, vp_tmin, vp_dinv, vp_npts, 0),
+                              make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
+                              (noise1 == nullptr && noise_scale != 0.0) ? 1 : 0, vac_count,
+                              (float)(vac_esc * vac_esc), nullptr, as_stream(stream));
 }
 
 extern "C" int tip_rk2_step_table_c32(
@@ -208,15 +259,18 @@ extern "C" int tip_rk2_step_table_c32(
     hipStream_t st = as_stream(stream);
     TIP_REQUIRE(phi1_tmp && phi2_tmp && pi1_tmp && pi2_tmp, "rk2 c32: tmp arrays are required");
     TIP_REQUIRE(vp_table, "rk2 c32: NULL table");
-    TIP_REQUIRE(vp_npts >= 2 && vp_npts < (int64_t(1) << 30), "rk2 c32: vp_npts out of range");
+    int rc = check_npts(vp_npts);
+    if (rc) return rc;
     float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
     TIP_REQUIRE(ab != nullptr, "rk2 c32: scratch allocation failed");
-    int rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
+    rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
     if (rc) return rc;
-    rc = launch_complex_f32(phi1, phi2, pi1, pi2, phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, dt, dx,
-                            eta_eff, mu, inv_a2, noise1, noise2, ab, vp_tmin, vp_dinv, vp_npts,
-                            zn_n, zn_dv, zn_active, nx, ny, nz, noise_seed, noise_scale,
-                            vac_count, vac_esc, st);
+    rc = launch_complex_f32(phi1, phi2, pi1, pi2, phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp,
+                            make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), noise1,
+                            noise2, make_table32(ab, vp_tmin, vp_dinv, vp_npts, 0),
+                            make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
+                            (noise1 == nullptr && noise_scale != 0.0) ? 1 : 0, vac_count,
+                            (float)(vac_esc * vac_esc), nullptr, st);
     if (rc) return rc;
     size_t bytes = (size_t)(nx * ny * nz) * sizeof(float);
     TIP_HIP(hipMemcpyAsync(phi1, phi1_tmp, bytes, hipMemcpyDeviceToDevice, st));

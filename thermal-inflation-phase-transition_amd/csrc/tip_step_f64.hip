@@ -33,14 +33,16 @@ int launch_real_f64(const double* phi, const double* pi, double* phi_out, double
                 (long long)vp_npts);
     TIP_REQUIRE(phi != phi_out && pi != pi_out, "rk2 f64: output must not alias input");
     StepScalars<double> c = make_scalars<double>(dt, dx, eta_eff, mu, inv_a2, noise_scale);
-    StepGeometry g = step_geometry(nx, ny, nz, 0, 1);
-    dim3 block(64, 4);
-    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
-              (unsigned)((nx + g.cx - 1) / g.cx));
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    rc = resolve_geometry(nullptr, nx, ny, nz, 0, 1, g, tg, nb);
+    if (rc) return rc;
     int hash = (noise == nullptr && noise_scale != 0.0) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk2_real<double, TableF64, 4>), grid, block, 0, st, phi, pi, phi_out,
-                       pi_out, noise, c, make_table(vp_table, vp_tmin, vp_dinv, vp_npts, vp_odd),
-                       (int)nx, (int)ny, (int)nz, g.cx, noise_seed, hash, vac, vac_esc);
+    hipLaunchKernelGGL((k_rk2_real<double, TableF64, 4>), dim3(nb), dim3(64, 4), 0, st, phi, pi,
+                       phi_out, pi_out, noise, c,
+                       make_table(vp_table, vp_tmin, vp_dinv, vp_npts, vp_odd), (int)nx, (int)ny,
+                       (int)nz, tg, noise_seed, hash, vac, vac_esc);
     TIP_HIP(hipGetLastError());
     return TIP_OK;
 }
@@ -102,15 +104,16 @@ extern "C" int tip_rk2_step_table_c64(
     zn.dv = zn_dv;
     zn.active = (zn_active > 0.5 && zn_n > 0.5) ? 1 : 0;
     zn.ipow = 0;
-    StepGeometry g = step_geometry(nx, ny, nz, 1, 1);
-    dim3 block(64, 4);
-    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
-              (unsigned)((nx + g.cx - 1) / g.cx));
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    rc = resolve_geometry(nullptr, nx, ny, nz, 1, 1, g, tg, nb);
+    if (rc) return rc;
     int hash = (noise1 == nullptr && noise_scale != 0.0) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk2_complex<double, TableF64, 4>), grid, block, 0, st, phi1, phi2, pi1,
-                       pi2, phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, noise1, noise2, c,
+    hipLaunchKernelGGL((k_rk2_complex<double, TableF64, 4>), dim3(nb), dim3(64, 4), 0, st, phi1,
+                       phi2, pi1, pi2, phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, noise1, noise2, c,
                        make_table(vp_table, vp_tmin, vp_dinv, vp_npts, 0), zn, (int)nx, (int)ny,
-                       (int)nz, g.cx, noise_seed, hash, vac_count, vac_esc * vac_esc);
+                       (int)nz, tg, noise_seed, hash, vac_count, vac_esc * vac_esc);
     TIP_HIP(hipGetLastError());
     size_t bytes = (size_t)(nx * ny * nz) * sizeof(double);
     TIP_HIP(hipMemcpyAsync(phi1, phi1_tmp, bytes, hipMemcpyDeviceToDevice, st));

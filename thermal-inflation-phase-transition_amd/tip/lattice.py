@@ -65,6 +65,12 @@ class Lattice:
     def set_zn(self, n: float, dv: float, active: float) -> None:
         _lib.call("tip_lattice_set_zn", self._h, float(n), float(dv), float(active))
 
+    def set_geometry(self, rows_per_block: int = 0, planes_per_block: int = 0,
+                     xcd_remap: bool = True) -> None:
+        """Launch geometry of the float32 step kernel (0 = default)."""
+        _lib.call("tip_lattice_set_geometry", self._h, int(rows_per_block), int(planes_per_block),
+                  int(bool(xcd_remap)))
+
     # -- evolution -----------------------------------------------------------------------
     def step(self, nsteps: int, *, dt: float, dx: float, eta_eff: float, mu: float,
              inv_a2: float = 1.0, noise_scale: float = 0.0, seed: int = 0,
