@@ -100,6 +100,60 @@ def cpu_baseline(shape, table, tmin, dinv, prm, budget_s: float = 12.0):
                 ms_per_step=1e3 * el / steps)
 
 
+def ensemble_workload(npts: int = 8192, seed: int = 2024):
+    """64 realisations: 32 x 64^3, 24 x 96^3, 8 x 128^3; seeds 1..64; T0 ~ U[6770, 7350] GeV;
+    per-instance 8192-point float64 V' tables on [0, 1.5 phi_true(T0)] (odd)."""
+    from tip import potential as P
+    from tip.ensemble import config3_shapes
+
+    shapes = config3_shapes()
+    rng = np.random.default_rng(seed)
+    T0s = rng.uniform(6770.0, 7350.0, size=len(shapes))
+    p = P.ModelParams.real()
+    mu, dx_phys, dt_phys = 1000.0, 0.002, 2.5e-4
+    inst = []
+    for i, T0 in enumerate(T0s):
+        vt = P.thermal_vev(float(T0), p)
+        t, tmin, dinv = P.build_vprime_table(float(T0), 0.0, 1.5 * vt, npts, p)
+        eta = T0 / mu
+        prm = dict(dt=mu * dt_phys, dx=mu * dx_phys, eta_eff=eta, mu=mu, inv_a2=1.0,
+                   noise_scale=P.noise_scale(eta, float(T0), mu * dt_phys, mu, dx_phys))
+        inst.append((t, tmin, dinv, prm))
+    return shapes, list(range(1, len(shapes) + 1)), T0s, inst
+
+
+def bench_ensemble(steps: int = 200):
+    from tip import autotune
+    from tip.ensemble import Ensemble
+
+    shapes, seeds, T0s, inst = ensemble_workload()
+    ens = Ensemble(shapes, seeds)
+    for i, (t, tmin, dinv, prm) in enumerate(inst):
+        ens.set_table(i, t, tmin, dinv, odd=True)
+        ens.set_params(i, **prm)
+    tune = autotune.tune_ensemble(ens, steps=10, use_cache=False)
+    log(f"ensemble autotune best {tune['best']} {tune['us_per_step']:.1f} us/step")
+    rng = np.random.default_rng(0)
+    for i, s in enumerate(shapes):
+        ens.upload(i, "phi", 0.01 * rng.normal(size=s))
+        ens.upload(i, "pi", np.zeros(s))
+    sites = float(sum(int(np.prod(s)) for s in shapes))
+    out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"]}
+    for mode in ("graph", "batched", "sequential"):
+        ens.step(10, mode=mode, timed=True)
+        t0 = time.perf_counter()
+        ms = ens.step(steps, mode=mode, timed=True)
+        wall = time.perf_counter() - t0
+        out[mode] = {"site_updates_per_s": sites * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "event_ms_per_step": ms / steps}
+        log(f"ensemble {mode}: {1e3 * wall / steps:.3f} ms/step")
+    out["graph_vs_sequential"] = (out["graph"]["site_updates_per_s"]
+                                  / out["sequential"]["site_updates_per_s"])
+    out["hbm_gbps_graph"] = 16.0 * out["graph"]["site_updates_per_s"] / 1e9
+    ens.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +161,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="real256", choices=["real256", "real256_f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -175,10 +230,13 @@ def main() -> int:
                      "kernel_us": kern_s * 1e6, "bytes_per_site": bps},
         "true_vacuum_fraction": frac_true,
     }
+    lat.close()
+    if not a.no_ensemble:
+        log("batched ensemble (64 realisations, 46.4M sites) ...")
+        out["ensemble"] = bench_ensemble()
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)
-    lat.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -242,6 +242,40 @@ int tip_lattice_vacuum_count(tip_lattice *lat, double esc, unsigned long long *c
 int tip_lattice_winding(tip_lattice *lat, int8_t *plaq_dev, unsigned long long *counts2);
 int tip_lattice_minmax(tip_lattice *lat, double *out2);
 
+/* ---- batched ensemble ---------------------------------------------------------------
+ * n independent real float32 realisations of uneven sizes (dims = n x {nx, ny, nz}), each with
+ * its own seed, V' table and step parameters, advanced together.  No counterpart in the
+ * reference, which runs one realisation per process; each instance follows rk2_step_table
+ * (simulation/latticeSimeRescale_numba.py:1875-1966) with hash noise
+ * noise_seed = tip_noise_seed(step, seed_i).
+ * Modes: TIP_ENS_SEQUENTIAL launches the single-lattice kernel once per instance per step
+ * (the plain loop); TIP_ENS_BATCHED launches one kernel per step over a work list of every
+ * instance's tiles; TIP_ENS_GRAPH captures the nsteps batched launches into a hipGraph once per
+ * (nsteps, buffer parity) and replays it.  All modes give identical fields. */
+#define TIP_ENS_SEQUENTIAL 0
+#define TIP_ENS_BATCHED 1
+#define TIP_ENS_GRAPH 2
+
+typedef struct tip_ensemble tip_ensemble;
+
+int tip_ensemble_create(tip_ensemble **out, int n, const int64_t *dims, const uint64_t *seeds);
+int tip_ensemble_destroy(tip_ensemble *ens);
+/* which: 0 = phi, 1 = pi; host float32 arrays of the instance's size */
+int tip_ensemble_upload(tip_ensemble *ens, int inst, int which, const float *host);
+int tip_ensemble_download(tip_ensemble *ens, int inst, int which, float *host);
+int tip_ensemble_set_table(tip_ensemble *ens, int inst, const double *host_table, int64_t npts,
+                           double tmin, double dinv, int odd);
+int tip_ensemble_set_params(tip_ensemble *ens, int inst, const tip_step_params *p);
+/* rows of 64 z-sites per block (1, 2, 4, 8), x planes per work item, XCD-aware block remap,
+ * and persistent_blocks > 0 to launch that many blocks each walking a contiguous item range
+ * (0 = one block per item). */
+int tip_ensemble_set_geometry(tip_ensemble *ens, int rows_per_block, int planes_per_block,
+                              int xcd_remap, int persistent_blocks);
+int tip_ensemble_step(tip_ensemble *ens, int64_t nsteps, int mode, float *elapsed_ms);
+int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long *host_counts);
+int tip_ensemble_step_count(tip_ensemble *ens, uint64_t *step);
+int tip_ensemble_set_step(tip_ensemble *ens, uint64_t step);
+
 #if defined(TIP_BUILDING) && defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

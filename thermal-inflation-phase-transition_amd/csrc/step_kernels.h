@@ -22,6 +22,7 @@ namespace tip {
 template <typename R>
 struct StepScalars {
     R dt, half_dt, inv_dx2, inv_mu2, eta, inv_a2, ns;
+    uint32_t word_mask;  // all ones (see unit_f32)
 };
 
 template <typename R>
@@ -36,18 +37,19 @@ inline StepScalars<R> make_scalars(double dt, double dx, double eta_eff, double 
     c.eta = (R)eta_eff;
     c.inv_a2 = (R)inv_a2;
     c.ns = (R)noise_scale;
+    c.word_mask = 0xFFFFFFFFu;
     return c;
 }
 
-__device__ __forceinline__ double gauss_of(double, uint64_t s) { return gauss_f64(s); }
-__device__ __forceinline__ float gauss_of(float, uint64_t s) { return gauss_f32(s); }
-__device__ __forceinline__ void gauss_pair_of(double, uint64_t s, double& a, double& b)
+__device__ __forceinline__ double gauss_of(double, uint64_t s, uint32_t) { return gauss_f64(s); }
+__device__ __forceinline__ float gauss_of(float, uint64_t s, uint32_t m) { return gauss_f32(s, m); }
+__device__ __forceinline__ void gauss_pair_of(double, uint64_t s, uint32_t, double& a, double& b)
 {
     gauss_pair_f64(s, a, b);
 }
-__device__ __forceinline__ void gauss_pair_of(float, uint64_t s, float& a, float& b)
+__device__ __forceinline__ void gauss_pair_of(float, uint64_t s, uint32_t m, float& a, float& b)
 {
-    gauss_pair_f32(s, a, b);
+    gauss_pair_f32(s, m, a, b);
 }
 
 __device__ __forceinline__ double r_abs(double x) { return fabs(x); }
@@ -62,93 +64,102 @@ __device__ __forceinline__ unsigned wave_count(bool p)
 // ---------------------------------------------------------------------------------------
 // Real field.
 // ---------------------------------------------------------------------------------------
-// One (z, y) tile of 64 x BY columns marching over planes [x0, x1).  Returns the wave's
-// count of sites with |phi_new| > esc when `count` is set (all 64 lanes of every wave must
-// call it with the same x0, x1 and count).
+// One tile of 64*BY consecutive sites of the flattened (y, z) plane, marching over x planes
+// [x0, x1).  Flattening keeps every lane busy whatever nz is (a 96-wide row would leave a
+// quarter of a 64 x k tile idle).  Lanes past the plane compute on site 0 and store nothing,
+// so loads need no predication.  Returns the wave's count of sites with |phi_new| > esc when
+// `count` is set (all lanes of every wave call it with the same x0, x1 and count).
 template <typename R, typename Tab, int BY>
 __device__ __forceinline__ unsigned rk2_real_tile(
     const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
     R* __restrict__ pi_out, const R* __restrict__ noise, const StepScalars<R>& c,
-    const Tab& tab, int nx, int ny, int nz, int tile_z, int tile_y, int x0, int x1,
+    const Tab& tab, int nx, int ny, int nz, int tile_p, int x0, int x1,
     uint64_t noise_seed, int hash_noise, bool count, R esc)
 {
-    const int k = tile_z * 64 + (int)threadIdx.x;
-    const int j = tile_y * BY + (int)threadIdx.y;
-    const bool act = (k < nz) && (j < ny);
-    const int kk = act ? k : 0, jj = act ? j : 0;
     const int nyz = ny * nz;
+    const int pl = tile_p * (64 * BY) + (int)threadIdx.y * 64 + (int)threadIdx.x;
+    const bool act = pl < nyz;
+    const int p0 = act ? pl : 0;
+    const int jj = p0 / nz, kk = p0 - jj * nz;
     const int kp = kk + 1 < nz ? kk + 1 : 0, km = kk - 1 >= 0 ? kk - 1 : nz - 1;
     const int jp = jj + 1 < ny ? jj + 1 : 0, jm = jj - 1 >= 0 ? jj - 1 : ny - 1;
-    const int row = jj * nz;
-    const int o_kp = row + kp, o_km = row + km, o_c = row + kk;
-    const int o_jp = jp * nz + kk, o_jm = jm * nz + kk;
+    // Byte offsets inside one x plane (< 2^32 by the dimension check).  Each load is a uniform
+    // plane base (scalar registers) plus a loop-invariant 32-bit lane offset, so the loop needs
+    // no per-load 64-bit address arithmetic.
+    const uint32_t E = (uint32_t)sizeof(R);
+    const uint32_t o_c = (uint32_t)(jj * nz + kk) * E;
+    const uint32_t o_kp = (uint32_t)(jj * nz + kp) * E, o_km = (uint32_t)(jj * nz + km) * E;
+    const uint32_t o_jp = (uint32_t)(jp * nz + kk) * E, o_jm = (uint32_t)(jm * nz + kk) * E;
+    const size_t plane = (size_t)nyz * E;
+    const uint32_t pb = (uint32_t)plane;  // plane bytes < 2^32 by the dimension check
+    const char* P = reinterpret_cast<const char*>(phi);
+    const char* Q = reinterpret_cast<const char*>(pi);
+    const char* PO = reinterpret_cast<const char*>(phi_out);
+    const char* QO = reinterpret_cast<const char*>(pi_out);
+    const char* NZ = reinterpret_cast<const char*>(noise);
+    auto at = [&](const char* b, int x) { return make_rsrc(b + (size_t)x * plane, pb); };
+    // Site seed uint64(site*K) ^ noise_seed with site = x*nyz + column: the product splits into
+    // a per-lane part and a per-plane part (exact mod 2^64).
+    const uint64_t colK = (uint64_t)(uint32_t)(jj * nz + kk) * (uint64_t)kNoiseMult;
+    const uint64_t planeK = (uint64_t)(uint32_t)nyz * (uint64_t)kNoiseMult;
+    uint64_t xK = (uint64_t)(uint32_t)x0 * planeK;
 
     unsigned cnt = 0;
-    R pm = 0, qm = 0, pc = 0, qc = 0;
-    if (act) {
-        const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
-        pm = phi[xm * nyz + o_c];
-        qm = pi[xm * nyz + o_c];
-        pc = phi[x0 * nyz + o_c];
-        qc = pi[x0 * nyz + o_c];
-    }
+    const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
+    R pm = buf_load<R>(at(P, xm), o_c);
+    R qm = buf_load<R>(at(Q, xm), o_c);
+    R pc = buf_load<R>(at(P, x0), o_c);
+    R qc = buf_load<R>(at(Q, x0), o_c);
     for (int x = x0; x < x1; ++x) {
         const int xp = x + 1 < nx ? x + 1 : 0;
-        const int base = x * nyz;
-        R pp = 0, qp = 0, pyp = 0, pym = 0, pzp = 0, pzm = 0, qyp = 0, qym = 0, qzp = 0, qzm = 0;
-        bool tv = false;
+        const __amdgpu_buffer_rsrc_t Pb = at(P, x), Qb = at(Q, x);
+        const R pp = buf_load<R>(at(P, xp), o_c);
+        const R qp = buf_load<R>(at(Q, xp), o_c);
+        const R pyp = buf_load<R>(Pb, o_jp), pym = buf_load<R>(Pb, o_jm);
+        const R pzp = buf_load<R>(Pb, o_kp), pzm = buf_load<R>(Pb, o_km);
+        const R qyp = buf_load<R>(Qb, o_jp), qym = buf_load<R>(Qb, o_jm);
+        const R qzp = buf_load<R>(Qb, o_kp), qzm = buf_load<R>(Qb, o_km);
+
+        // Stage 1 (reference pass 1): k1 at (phi, pi); midpoint (phi_t, pi_t).
+        R s6 = pp + pm;
+        s6 = s6 + pyp;
+        s6 = s6 + pym;
+        s6 = s6 + pzp;
+        s6 = s6 + pzm;
+        R lap = (s6 - (R)6 * pc) * c.inv_dx2;
+        R dV = tab(pc);
+        R k1 = c.inv_a2 * lap - c.eta * qc - dV * c.inv_mu2;
+        R ftc = pc + c.half_dt * qc;
+        R ptc = qc + c.half_dt * k1;
+
+        // Stage 2 (reference pass 2): phi_t of the six neighbours is local to each of them.
+        R t_ip = pp + c.half_dt * qp;
+        R t_im = pm + c.half_dt * qm;
+        R t_jp = pyp + c.half_dt * qyp;
+        R t_jm = pym + c.half_dt * qym;
+        R t_kp = pzp + c.half_dt * qzp;
+        R t_km = pzm + c.half_dt * qzm;
+        R s6b = t_ip + t_im;
+        s6b = s6b + t_jp;
+        s6b = s6b + t_jm;
+        s6b = s6b + t_kp;
+        s6b = s6b + t_km;
+        R lap2 = (s6b - (R)6 * ftc) * c.inv_dx2;
+        R dV2 = tab(ftc);
+        R k2 = c.inv_a2 * lap2 - c.eta * ptc - dV2 * c.inv_mu2;
+
+        R nz_ = 0;
+        if (noise) nz_ = buf_load<R>(at(NZ, x), o_c);
+        else if (hash_noise) nz_ = c.ns * gauss_of((R)0, (colK + xK) ^ noise_seed, c.word_mask);
+        R pn = pc + c.dt * ptc;
+        R qn = qc + (c.dt * k2 + nz_);
         if (act) {
-            pp = phi[xp * nyz + o_c];
-            qp = pi[xp * nyz + o_c];
-            pyp = phi[base + o_jp];
-            pym = phi[base + o_jm];
-            pzp = phi[base + o_kp];
-            pzm = phi[base + o_km];
-            qyp = pi[base + o_jp];
-            qym = pi[base + o_jm];
-            qzp = pi[base + o_kp];
-            qzm = pi[base + o_km];
-
-            // Stage 1 (reference pass 1): k1 at (phi, pi); midpoint (phi_t, pi_t).
-            R s6 = pp + pm;
-            s6 = s6 + pyp;
-            s6 = s6 + pym;
-            s6 = s6 + pzp;
-            s6 = s6 + pzm;
-            R lap = (s6 - (R)6 * pc) * c.inv_dx2;
-            R dV = tab(pc);
-            R k1 = c.inv_a2 * lap - c.eta * qc - dV * c.inv_mu2;
-            R ftc = pc + c.half_dt * qc;
-            R ptc = qc + c.half_dt * k1;
-
-            // Stage 2 (reference pass 2): phi_t of the six neighbours is local to each of them.
-            R t_ip = pp + c.half_dt * qp;
-            R t_im = pm + c.half_dt * qm;
-            R t_jp = pyp + c.half_dt * qyp;
-            R t_jm = pym + c.half_dt * qym;
-            R t_kp = pzp + c.half_dt * qzp;
-            R t_km = pzm + c.half_dt * qzm;
-            R s6b = t_ip + t_im;
-            s6b = s6b + t_jp;
-            s6b = s6b + t_jm;
-            s6b = s6b + t_kp;
-            s6b = s6b + t_km;
-            R lap2 = (s6b - (R)6 * ftc) * c.inv_dx2;
-            R dV2 = tab(ftc);
-            R k2 = c.inv_a2 * lap2 - c.eta * ptc - dV2 * c.inv_mu2;
-
-            const int s = base + o_c;
-            R nz_ = 0;
-            if (noise) nz_ = noise[s];
-            else if (hash_noise) nz_ = c.ns * gauss_of((R)0, site_seed((int64_t)s, noise_seed));
-            R pn = pc + c.dt * ptc;
-            R qn = qc + (c.dt * k2 + nz_);
-            phi_out[s] = pn;
-            pi_out[s] = qn;
-            tv = r_abs(pn) > esc;
+            buf_store(pn, at(PO, x), o_c);
+            buf_store(qn, at(QO, x), o_c);
         }
-        if (count) cnt += wave_count(tv);
+        if (count) cnt += wave_count(act && r_abs(pn) > esc);
         pm = pc; qm = qc; pc = pp; qc = qp;
+        xK += planeK;
     }
     return cnt;
 }
@@ -159,14 +170,12 @@ __device__ __forceinline__ unsigned rk2_real_tile(
 // through one contiguous range of x chunks and the neighbour planes a chunk re-reads stay in
 // that XCD's L2.  Requires nb % 8 == 0 (the launcher clears the flag otherwise).
 struct TileGrid {
-    int tiles_z, tiles_y, chunks, cx, xcd_remap;
-    __device__ __forceinline__ void decode(int b, int nb, int& tz, int& ty, int& ch) const
+    int tiles, chunks, cx, xcd_remap;  // tiles: plane tiles of 64*BY sites
+    __device__ __forceinline__ void decode(int b, int nb, int& tp, int& ch) const
     {
         if (xcd_remap) b = (b & 7) * (nb >> 3) + (b >> 3);
-        tz = b % tiles_z;
-        int r = b / tiles_z;
-        ty = r % tiles_y;
-        ch = r / tiles_y;
+        tp = b % tiles;
+        ch = b / tiles;
     }
 };
 
@@ -177,13 +186,13 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_real(
     int ny, int nz, TileGrid g, uint64_t noise_seed, int hash_noise,
     unsigned long long* __restrict__ vac, R esc)
 {
-    int tz, ty, ch;
-    g.decode((int)blockIdx.x, (int)gridDim.x, tz, ty, ch);
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
     unsigned cnt = rk2_real_tile<R, Tab, BY>(phi, pi, phi_out, pi_out, noise, c, tab, nx, ny, nz,
-                                             tz, ty, x0, x1, noise_seed, hash_noise,
-                                             vac != nullptr, esc);
+                                             tp, x0, x1, noise_seed, hash_noise, vac != nullptr,
+                                             esc);
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
@@ -248,15 +257,15 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
     int nz, TileGrid g, uint64_t noise_seed, int hash_noise,
     unsigned long long* __restrict__ vac, R esc2)
 {
-    int tz, ty, ch;
-    g.decode((int)blockIdx.x, (int)gridDim.x, tz, ty, ch);
-    const int k = tz * 64 + (int)threadIdx.x;
-    const int j = ty * BY + (int)threadIdx.y;
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    const bool act = (k < nz) && (j < ny);
-    const int kk = act ? k : 0, jj = act ? j : 0;
     const int nyz = ny * nz;
+    const int pl = tp * (64 * BY) + (int)threadIdx.y * 64 + (int)threadIdx.x;
+    const bool act = pl < nyz;
+    const int p0 = act ? pl : 0;
+    const int jj = p0 / nz, kk = p0 - jj * nz;
     const int kp = kk + 1 < nz ? kk + 1 : 0, km = kk - 1 >= 0 ? kk - 1 : nz - 1;
     const int jp = jj + 1 < ny ? jj + 1 : 0, jm = jj - 1 >= 0 ? jj - 1 : ny - 1;
     const int row = jj * nz;
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
                 n2 = noise2 ? noise2[s] : (R)0;
             } else if (hash_noise) {
                 R z1, z2;
-                gauss_pair_of((R)0, site_seed((int64_t)s, noise_seed), z1, z2);
+                gauss_pair_of((R)0, site_seed((int64_t)s, noise_seed), c.word_mask, z1, z2);
                 n1 = c.ns * z1;
                 n2 = c.ns * z2;
             }
@@ -371,17 +380,17 @@ int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz
 
 // Launchers shared by the C-ABI entry points, the lattice and the ensemble.
 int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out,
-                    const StepScalars<float>& c, const float* noise, const TableF32& tab,
+                    const StepScalars<float>& c, const float* noise, const SegTable& tab,
                     int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
                     unsigned long long* vac, float esc, const StepGeometry* geom,
                     hipStream_t st);
 int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
                        float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
-                       const float* noise1, const float* noise2, const TableF32& tab,
+                       const float* noise1, const float* noise2, const SegTable& tab,
                        const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
                        uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
                        const StepGeometry* geom, hipStream_t st);
-TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd);
+SegTable make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd);
 ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active);
 
 }  // namespace tip

@@ -33,6 +33,9 @@ int check_dims(int64_t nx, int64_t ny, int64_t nz)
     TIP_REQUIRE(nx <= 65535 && ny <= 65535 && nz <= 65535 && nx * ny * nz < kMaxSites,
                 "lattice %lldx%lldx%lld exceeds 2^31 sites", (long long)nx, (long long)ny,
                 (long long)nz);
+    // byte offsets inside one x plane are 32-bit in the step kernels (float64: 8 B/site)
+    TIP_REQUIRE(ny * nz < (int64_t(1) << 29), "lattice plane %lldx%lld exceeds 2^29 sites",
+                (long long)ny, (long long)nz);
     return TIP_OK;
 }
 

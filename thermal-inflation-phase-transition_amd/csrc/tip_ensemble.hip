@@ -1,0 +1,473 @@
+// tip_ensemble.hip -- many independent real-field realisations (uneven lattice sizes, seeds,
+// temperatures and V' tables) advanced together.
+//
+// The reference runs one realisation per process (SURVEY: no ensemble layer exists); the
+// nucleation-rate and string statistics need ensembles.  Here every instance's float32 fields
+// live in one HBM pool, and one kernel launch per RK2 step covers all instances: the host
+// builds a work list of (instance, plane tile, x chunk) items and each block runs the same
+// tile body as the single-lattice kernel on its item.  Instance parameters, table pointers and
+// buffers sit in a device descriptor array, and the noise step counter is a device word, so a
+// K-step loop (K launches + one counter bump) is captured once into a hipGraph and replayed;
+// parameter updates rewrite the descriptors and leave the graph valid.
+#include <map>
+#include <vector>
+
+#include "step_kernels.h"
+
+using namespace tip;
+
+namespace {
+
+struct EnsInst {
+    float* f[2][2];  // [parity][phi, pi]
+    const float2* ab;
+    StepScalars<float> c;
+    float tmin, dinv;
+    int last, odd;
+    int nx, ny, nz;
+    uint64_t seed_mix;
+    int hash;
+};
+
+// Work item: instance, plane tile (64*BY consecutive (y, z) sites), first x plane; `cx`
+// planes per item.
+struct EnsItem {
+    int inst, tp, x0, pad;
+};
+
+template <int BY, bool ODD>
+__global__ __launch_bounds__(64 * BY) void k_rk2_ensemble(
+    const EnsInst* __restrict__ insts, const EnsItem* __restrict__ items, int nitems, int cx,
+    int parity, const uint64_t* __restrict__ step_base, int step_off, int xcd_remap,
+    unsigned long long* __restrict__ vac, float esc)
+{
+    int b = (int)blockIdx.x;
+    const int nb = (int)gridDim.x;
+    if (xcd_remap) b = (b & 7) * (nb >> 3) + (b >> 3);
+    // persistent blocks: block b owns the contiguous item range [b*q, min((b+1)*q, nitems))
+    const int q = (nitems + nb - 1) / nb;
+    const int i0 = b * q;
+    const int i1 = min(i0 + q, nitems);
+    const uint64_t step = step_base[0] + (uint64_t)step_off;
+    for (int i = i0; i < i1; ++i) {
+        const EnsItem it = items[i];
+        const EnsInst& I = insts[it.inst];
+        TableF32<ODD> tab;
+        tab.ab = I.ab;
+        tab.tmin = I.tmin;
+        tab.dinv = I.dinv;
+        tab.last = I.last;
+        const int x1 = min(it.x0 + cx, I.nx);
+        unsigned cnt = rk2_real_tile<float, TableF32<ODD>, BY>(
+            I.f[parity][0], I.f[parity][1], I.f[parity ^ 1][0], I.f[parity ^ 1][1], nullptr, I.c,
+            tab, I.nx, I.ny, I.nz, it.tp, it.x0, x1, step ^ I.seed_mix, I.hash,
+            vac != nullptr, esc);
+        if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
+    }
+}
+
+__global__ void k_advance(uint64_t* step_base, int n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) step_base[0] += (uint64_t)n;
+}
+
+}  // namespace
+
+struct tip_ensemble {
+    int n = 0;
+    std::vector<int64_t> dims;   // 3n
+    std::vector<uint64_t> seeds;
+    std::vector<size_t> off;     // element offset of instance i in each pool quarter
+    size_t total = 0;            // sites over all instances
+    float* pool = nullptr;       // 4 quarters: phi0, pi0, phi1, pi1 (each `total` floats)
+    std::vector<EnsInst> host_inst;
+    EnsInst* dev_inst = nullptr;
+    std::vector<double*> table64;
+    std::vector<float2*> table_ab;
+    std::vector<int64_t> table_cap;
+    std::vector<int> has_table, has_params;
+    uint64_t* dev_step = nullptr;
+    uint64_t step = 0;
+    int parity = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* dev_vac = nullptr;
+    // geometry
+    int by = 4, cx = 8, xcd = 1, persistent = 0;
+    EnsItem* dev_items = nullptr;
+    int nitems = 0;
+    bool items_dirty = true;
+    bool desc_dirty = true;
+    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;  // (nsteps, parity) -> exec
+};
+
+namespace {
+
+void drop_graphs(tip_ensemble* E)
+{
+    for (auto& kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
+    E->graphs.clear();
+}
+
+int release(tip_ensemble* E)
+{
+    if (!E) return TIP_OK;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    drop_graphs(E);
+    if (E->pool) (void)hipFree(E->pool);
+    if (E->dev_inst) (void)hipFree(E->dev_inst);
+    if (E->dev_items) (void)hipFree(E->dev_items);
+    if (E->dev_step) (void)hipFree(E->dev_step);
+    if (E->dev_vac) (void)hipFree(E->dev_vac);
+    for (auto p : E->table64)
+        if (p) (void)hipFree(p);
+    for (auto p : E->table_ab)
+        if (p) (void)hipFree(p);
+    if (E->ev0) (void)hipEventDestroy(E->ev0);
+    if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+    return TIP_OK;
+}
+
+size_t inst_sites(const tip_ensemble* E, int i)
+{
+    return (size_t)(E->dims[3 * i] * E->dims[3 * i + 1] * E->dims[3 * i + 2]);
+}
+
+float* field(tip_ensemble* E, int i, int parity, int which)
+{
+    // quarters: [parity*2 + which]
+    return E->pool + (size_t)(parity * 2 + which) * E->total + E->off[i];
+}
+
+void fill_pointers(tip_ensemble* E)
+{
+    for (int i = 0; i < E->n; ++i) {
+        EnsInst& I = E->host_inst[i];
+        for (int p = 0; p < 2; ++p)
+            for (int w = 0; w < 2; ++w) I.f[p][w] = field(E, i, p, w);
+        I.nx = (int)E->dims[3 * i];
+        I.ny = (int)E->dims[3 * i + 1];
+        I.nz = (int)E->dims[3 * i + 2];
+        I.seed_mix = E->seeds[i] * 0x9E3779B97F4A7C15ULL;
+    }
+}
+
+int sync_descriptors(tip_ensemble* E)
+{
+    if (!E->desc_dirty) return TIP_OK;
+    TIP_HIP(hipMemcpyAsync(E->dev_inst, E->host_inst.data(), sizeof(EnsInst) * E->n,
+                           hipMemcpyHostToDevice, E->stream));
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    E->desc_dirty = false;
+    return TIP_OK;
+}
+
+int build_items(tip_ensemble* E)
+{
+    if (!E->items_dirty) return TIP_OK;
+    std::vector<EnsItem> items;
+    for (int i = 0; i < E->n; ++i) {
+        const int nx = (int)E->dims[3 * i], ny = (int)E->dims[3 * i + 1], nz = (int)E->dims[3 * i + 2];
+        const int tiles = (ny * nz + 64 * E->by - 1) / (64 * E->by);
+        for (int x0 = 0; x0 < nx; x0 += E->cx)
+            for (int t = 0; t < tiles; ++t) items.push_back({i, t, x0, 0});
+    }
+    TIP_REQUIRE(!items.empty() && items.size() < (size_t(1) << 30), "ensemble: bad work list");
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    if (E->dev_items) TIP_HIP(hipFree(E->dev_items));
+    E->dev_items = nullptr;
+    TIP_HIP(hipMalloc(&E->dev_items, sizeof(EnsItem) * items.size()));
+    TIP_HIP(hipMemcpy(E->dev_items, items.data(), sizeof(EnsItem) * items.size(),
+                      hipMemcpyHostToDevice));
+    E->nitems = (int)items.size();
+    E->items_dirty = false;
+    drop_graphs(E);
+    return TIP_OK;
+}
+
+int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long* vac, float esc)
+{
+    int nb = E->nitems;
+    if (E->persistent > 0 && E->persistent < nb) nb = E->persistent;
+    const int remap = (E->xcd && nb % 8 == 0) ? 1 : 0;
+#define TIP_LAUNCH_ENS(BYV, ODD)                                                                \
+    hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, E->stream,     \
+                       E->dev_inst, E->dev_items, E->nitems, E->cx, parity, E->dev_step,       \
+                       step_off, remap, vac, esc)
+#define TIP_LAUNCH_ENS_BY(ODD)                         \
+    switch (E->by) {                                   \
+        case 1: TIP_LAUNCH_ENS(1, ODD); break;         \
+        case 2: TIP_LAUNCH_ENS(2, ODD); break;         \
+        case 8: TIP_LAUNCH_ENS(8, ODD); break;         \
+        default: TIP_LAUNCH_ENS(4, ODD); break;        \
+    }
+    if (E->host_inst[0].odd) {
+        TIP_LAUNCH_ENS_BY(true)
+    } else {
+        TIP_LAUNCH_ENS_BY(false)
+    }
+#undef TIP_LAUNCH_ENS_BY
+#undef TIP_LAUNCH_ENS
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+int ready(tip_ensemble* E)
+{
+    for (int i = 0; i < E->n; ++i) {
+        TIP_REQUIRE(E->has_table[i], "ensemble: instance %d has no V' table", i);
+        TIP_REQUIRE(E->host_inst[i].odd == E->host_inst[0].odd,
+                    "ensemble: all instances must use odd tables or all even ones");
+        TIP_REQUIRE(E->has_params[i], "ensemble: instance %d has no step parameters", i);
+    }
+    int rc = sync_descriptors(E);
+    if (rc) return rc;
+    return build_items(E);
+}
+
+}  // namespace
+
+extern "C" int tip_ensemble_create(tip_ensemble** out, int n, const int64_t* dims,
+                                   const uint64_t* seeds)
+{
+    TIP_REQUIRE(out && dims, "tip_ensemble_create: NULL");
+    *out = nullptr;
+    TIP_REQUIRE(n >= 1 && n <= 65536, "tip_ensemble_create: n=%d out of range", n);
+    tip_ensemble* E = new tip_ensemble();
+    E->n = n;
+    E->dims.assign(dims, dims + 3 * (size_t)n);
+    E->seeds.resize(n);
+    E->off.resize(n);
+    for (int i = 0; i < n; ++i) {
+        int rc = check_dims(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+        if (rc) {
+            delete E;
+            return rc;
+        }
+        E->seeds[i] = seeds ? seeds[i] : (uint64_t)i;
+        E->off[i] = E->total;
+        // keep every instance 256-byte aligned inside the pool
+        E->total += (inst_sites(E, i) + 63) & ~size_t(63);
+    }
+    E->host_inst.assign(n, EnsInst{});
+    E->table64.assign(n, nullptr);
+    E->table_ab.assign(n, nullptr);
+    E->table_cap.assign(n, 0);
+    E->has_table.assign(n, 0);
+    E->has_params.assign(n, 0);
+    hipError_t e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev1);
+    if (e == hipSuccess) e = hipMalloc(&E->pool, 4 * E->total * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(E->pool, 0, 4 * E->total * sizeof(float), E->stream);
+    if (e == hipSuccess) e = hipMalloc(&E->dev_inst, sizeof(EnsInst) * n);
+    if (e == hipSuccess) e = hipMalloc(&E->dev_step, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemsetAsync(E->dev_step, 0, sizeof(uint64_t), E->stream);
+    if (e == hipSuccess) e = hipMalloc(&E->dev_vac, sizeof(unsigned long long) * n);
+    if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    if (e != hipSuccess) {
+        int r = fail_hip(e, "tip_ensemble_create");
+        release(E);
+        return r;
+    }
+    fill_pointers(E);
+    *out = E;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_destroy(tip_ensemble* ens) { return release(ens); }
+
+extern "C" int tip_ensemble_upload(tip_ensemble* E, int i, int which, const float* host)
+{
+    TIP_REQUIRE(E && host, "tip_ensemble_upload: NULL");
+    TIP_REQUIRE(i >= 0 && i < E->n && (which == 0 || which == 1), "tip_ensemble_upload: bad index");
+    TIP_HIP(hipMemcpyAsync(field(E, i, E->parity, which), host, inst_sites(E, i) * sizeof(float),
+                           hipMemcpyHostToDevice, E->stream));
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_download(tip_ensemble* E, int i, int which, float* host)
+{
+    TIP_REQUIRE(E && host, "tip_ensemble_download: NULL");
+    TIP_REQUIRE(i >= 0 && i < E->n && (which == 0 || which == 1),
+                "tip_ensemble_download: bad index");
+    TIP_HIP(hipMemcpyAsync(host, field(E, i, E->parity, which), inst_sites(E, i) * sizeof(float),
+                           hipMemcpyDeviceToHost, E->stream));
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_set_table(tip_ensemble* E, int i, const double* host_table,
+                                      int64_t npts, double tmin, double dinv, int odd)
+{
+    TIP_REQUIRE(E && host_table, "tip_ensemble_set_table: NULL");
+    TIP_REQUIRE(i >= 0 && i < E->n, "tip_ensemble_set_table: bad instance %d", i);
+    TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "tip_ensemble_set_table: npts out of range");
+    TIP_REQUIRE(dinv > 0.0, "tip_ensemble_set_table: dinv must be positive");
+    if (npts > E->table_cap[i]) {
+        TIP_HIP(hipStreamSynchronize(E->stream));
+        if (E->table64[i]) TIP_HIP(hipFree(E->table64[i]));
+        if (E->table_ab[i]) TIP_HIP(hipFree(E->table_ab[i]));
+        E->table64[i] = nullptr;
+        E->table_ab[i] = nullptr;
+        TIP_HIP(hipMalloc(&E->table64[i], npts * sizeof(double)));
+        TIP_HIP(hipMalloc(&E->table_ab[i], (npts - 1) * sizeof(float2)));
+        E->table_cap[i] = npts;
+    }
+    TIP_HIP(hipMemcpyAsync(E->table64[i], host_table, npts * sizeof(double),
+                           hipMemcpyHostToDevice, E->stream));
+    int rc = prepare_table_f32(E->table64[i], npts, tmin, dinv, E->table_ab[i], E->stream);
+    if (rc) return rc;
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    EnsInst& I = E->host_inst[i];
+    I.ab = E->table_ab[i];
+    I.tmin = (float)tmin;
+    I.dinv = (float)dinv;
+    I.last = (int)(npts - 2);
+    I.odd = odd ? 1 : 0;
+    E->has_table[i] = 1;
+    E->desc_dirty = true;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_set_params(tip_ensemble* E, int i, const tip_step_params* p)
+{
+    TIP_REQUIRE(E && p, "tip_ensemble_set_params: NULL");
+    TIP_REQUIRE(i >= 0 && i < E->n, "tip_ensemble_set_params: bad instance %d", i);
+    EnsInst& I = E->host_inst[i];
+    I.c = make_scalars<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale);
+    I.hash = p->noise_scale != 0.0 ? 1 : 0;
+    E->has_params[i] = 1;
+    E->desc_dirty = true;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_set_geometry(tip_ensemble* E, int rows_per_block,
+                                         int planes_per_block, int xcd_remap,
+                                         int persistent_blocks)
+{
+    TIP_REQUIRE(E, "tip_ensemble_set_geometry: NULL");
+    TIP_REQUIRE(rows_per_block == 1 || rows_per_block == 2 || rows_per_block == 4 ||
+                    rows_per_block == 8,
+                "ensemble geometry: rows per block must be 1, 2, 4 or 8");
+    TIP_REQUIRE(planes_per_block >= 1 && planes_per_block <= 4096,
+                "ensemble geometry: planes per block out of range");
+    TIP_REQUIRE(persistent_blocks >= 0, "ensemble geometry: persistent_blocks < 0");
+    E->by = rows_per_block;
+    E->cx = planes_per_block;
+    E->xcd = xcd_remap ? 1 : 0;
+    E->persistent = persistent_blocks;
+    E->items_dirty = true;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, float* elapsed_ms)
+{
+    TIP_REQUIRE(E, "tip_ensemble_step: NULL");
+    TIP_REQUIRE(nsteps >= 0, "tip_ensemble_step: nsteps < 0");
+    TIP_REQUIRE(mode == TIP_ENS_SEQUENTIAL || mode == TIP_ENS_BATCHED || mode == TIP_ENS_GRAPH,
+                "tip_ensemble_step: unknown mode %d", mode);
+    int rc = ready(E);
+    if (rc) return rc;
+    if (elapsed_ms) TIP_HIP(hipEventRecord(E->ev0, E->stream));
+    if (mode == TIP_ENS_SEQUENTIAL) {
+        // the plain loop: one single-lattice kernel launch per instance per step
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int par = E->parity;
+            for (int i = 0; i < E->n; ++i) {
+                const EnsInst& I = E->host_inst[i];
+                SegTable tab;
+                tab.ab = I.ab;
+                tab.tmin = I.tmin;
+                tab.dinv = I.dinv;
+                tab.last = I.last;
+                tab.odd = I.odd;
+                rc = launch_real_f32(I.f[par][0], I.f[par][1], I.f[par ^ 1][0], I.f[par ^ 1][1],
+                                     I.c, nullptr, tab, I.nx, I.ny, I.nz,
+                                     (E->step + (uint64_t)s) ^ I.seed_mix, I.hash, nullptr, 0.0f,
+                                     nullptr, E->stream);
+                if (rc) return rc;
+            }
+            E->parity ^= 1;
+        }
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
+        TIP_HIP(hipGetLastError());
+    } else if (mode == TIP_ENS_BATCHED || nsteps == 0) {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            rc = launch_batched(E, E->parity, (int)s, nullptr, 0.0f);
+            if (rc) return rc;
+            E->parity ^= 1;
+        }
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
+        TIP_HIP(hipGetLastError());
+    } else {
+        TIP_REQUIRE(nsteps <= 100000, "tip_ensemble_step: graph of more than 1e5 steps");
+        auto key = std::make_pair(nsteps, E->parity);
+        auto itg = E->graphs.find(key);
+        if (itg == E->graphs.end()) {
+            hipGraph_t g;
+            TIP_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+            int par = E->parity;
+            for (int64_t s = 0; s < nsteps; ++s) {
+                rc = launch_batched(E, par, (int)s, nullptr, 0.0f);
+                if (rc) {
+                    (void)hipStreamEndCapture(E->stream, &g);
+                    return rc;
+                }
+                par ^= 1;
+            }
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step,
+                               (int)nsteps);
+            TIP_HIP(hipStreamEndCapture(E->stream, &g));
+            hipGraphExec_t ex;
+            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate");
+            itg = E->graphs.emplace(key, ex).first;
+        }
+        TIP_HIP(hipGraphLaunch(itg->second, E->stream));
+        if (nsteps & 1) E->parity ^= 1;
+    }
+    E->step += (uint64_t)nsteps;
+    if (elapsed_ms) {
+        TIP_HIP(hipEventRecord(E->ev1, E->stream));
+        TIP_HIP(hipEventSynchronize(E->ev1));
+        TIP_HIP(hipEventElapsedTime(elapsed_ms, E->ev0, E->ev1));
+    }
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_vacuum_counts(tip_ensemble* E, double esc,
+                                          unsigned long long* host_counts)
+{
+    TIP_REQUIRE(E && host_counts, "tip_ensemble_vacuum_counts: NULL");
+    TIP_HIP(hipMemsetAsync(E->dev_vac, 0, sizeof(unsigned long long) * E->n, E->stream));
+    for (int i = 0; i < E->n; ++i) {
+        int rc = tip_vacuum_count_f32(field(E, i, E->parity, 0), (int64_t)inst_sites(E, i), esc,
+                                      E->dev_vac + i, E->stream);
+        if (rc) return rc;
+    }
+    TIP_HIP(hipMemcpyAsync(host_counts, E->dev_vac, sizeof(unsigned long long) * E->n,
+                           hipMemcpyDeviceToHost, E->stream));
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_step_count(tip_ensemble* E, uint64_t* step)
+{
+    TIP_REQUIRE(E && step, "tip_ensemble_step_count: NULL");
+    *step = E->step;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_set_step(tip_ensemble* E, uint64_t step)
+{
+    TIP_REQUIRE(E, "tip_ensemble_set_step: NULL");
+    TIP_HIP(hipStreamSynchronize(E->stream));
+    TIP_HIP(hipMemcpy(E->dev_step, &step, sizeof(uint64_t), hipMemcpyHostToDevice));
+    E->step = step;
+    return TIP_OK;
+}

@@ -104,34 +104,74 @@ __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double
     zs = r * s;
 }
 
-// float32 Box-Muller from the same exact (u1, u2): u's rounded to float32, then the hardware
+// float32 Box-Muller from the same hash states: u's in float32, then the hardware
 // transcendentals.  v_cos/v_sin take their argument in revolutions, so cos(2*pi*u2) is one
 // instruction; -2 ln u1 = -2 ln2 * log2 u1.
-__device__ __forceinline__ void units_f32(uint64_t seed, float& u1f, float& u2f)
+// (x >> 11) * 2^-53 in float32 without a float64 detour: the high word converts exactly up to
+// 24 bits and the (truncated) low word adds the remaining fraction; the result is within one
+// float32 ulp of the rounded float64 value.  `word_mask` is a runtime all-ones word: ANDing
+// the high word with it stops the compiler from re-widening the 32-bit conversion into a
+// multi-instruction 64-bit one (it otherwise proves the word came from a 64-bit shift).
+__device__ __forceinline__ float unit_f32(uint64_t x, uint32_t word_mask)
+{
+    const uint32_t hi = (uint32_t)(x >> 32) & word_mask;
+    const uint32_t lo = (uint32_t)x & 0xFFFFF800u;
+    return fmaf((float)lo, 0x1p-64f, (float)hi * 0x1p-32f);
+}
+
+__device__ __forceinline__ float gauss_f32(uint64_t seed, uint32_t word_mask)
 {
     uint64_t x1, x2;
     hash_states(seed, x1, x2);
-    double u1 = hash_unit(x1);
-    if (u1 < 1e-12) u1 = 1e-12;
-    u1f = (float)u1;
-    u2f = (float)hash_unit(x2);
+    float u1 = fmaxf(unit_f32(x1, word_mask), 1e-12f);
+    float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    return r * __builtin_amdgcn_cosf(unit_f32(x2, word_mask));
 }
 
-__device__ __forceinline__ float gauss_f32(uint64_t seed)
+__device__ __forceinline__ void gauss_pair_f32(uint64_t seed, uint32_t word_mask, float& zc,
+                                               float& zs)
 {
-    float u1f, u2f;
-    units_f32(seed, u1f, u2f);
-    float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1f));
-    return r * __builtin_amdgcn_cosf(u2f);
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    float u1 = fmaxf(unit_f32(x1, word_mask), 1e-12f);
+    float u2 = unit_f32(x2, word_mask);
+    float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    zc = r * __builtin_amdgcn_cosf(u2);
+    zs = r * __builtin_amdgcn_sinf(u2);
 }
 
-__device__ __forceinline__ void gauss_pair_f32(uint64_t seed, float& zc, float& zs)
+// ---------------------------------------------------------------------------------------
+// Raw buffer access.  A buffer resource (base, byte range) lives in scalar registers; a load
+// is one instruction with a 32-bit per-lane byte offset and no vector address arithmetic.
+// 0x00020000 is the GFX9-family (gfx950) raw-buffer word 3 (32-bit data format).
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes)
 {
-    float u1f, u2f;
-    units_f32(seed, u1f, u2f);
-    float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1f));
-    zc = r * __builtin_amdgcn_cosf(u2f);
-    zs = r * __builtin_amdgcn_sinf(u2f);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+template <typename R>
+__device__ __forceinline__ R buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    using u2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -167,20 +207,43 @@ struct TableF64 {
     }
 };
 
+// ODD is a compile-time flag so the even table pays nothing for the odd extension.  The
+// segment index is the truncation of fidx clamped to [0, last] in float: for the segment
+// choice this equals the reference's clamp of the truncated index (frac is not needed in
+// the A + B x form).
+template <bool ODD>
 struct TableF32 {
     const float2* ab;
     float tmin, dinv;
     int last;
-    int odd;
     __device__ __forceinline__ float operator()(float x) const
     {
-        float ax = odd ? fabsf(x) : x;
-        float fidx = (ax - tmin) * dinv;
-        int idx = (int)fminf(fmaxf(fidx, -1.0f), (float)last + 1.0f);
-        idx = idx < 0 ? 0 : (idx > last ? last : idx);
-        float2 s = ab[idx];
-        float v = fmaf(s.y, ax, s.x);
-        return (odd && x < 0.0f) ? -v : v;
+        const float ax = ODD ? fabsf(x) : x;
+        const float fidx = (ax - tmin) * dinv;
+        const int idx = (int)__builtin_amdgcn_fmed3f(fidx, 0.0f, (float)last);
+        const float2 s = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(ab) +
+                                                          (uint32_t)idx * 8u);
+        const float v = fmaf(s.y, ax, s.x);
+        return (ODD && x < 0.0f) ? -v : v;
+    }
+};
+
+// Host-side description of a float32 segment table (the ODD flag becomes a template argument
+// at launch).
+struct SegTable {
+    const float2* ab;
+    float tmin, dinv;
+    int last;
+    int odd;
+    template <bool ODD>
+    TableF32<ODD> as() const
+    {
+        TableF32<ODD> t;
+        t.ab = ab;
+        t.tmin = tmin;
+        t.dinv = dinv;
+        t.last = last;
+        return t;
     }
 };
 

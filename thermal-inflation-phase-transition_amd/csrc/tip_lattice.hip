@@ -199,7 +199,7 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
     TIP_REQUIRE(nsteps >= 0, "tip_lattice_step: nsteps < 0");
     const StepScalars<float> c32 =
         make_scalars<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale);
-    const TableF32 tab32 = make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd);
+    const SegTable tab32 = make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd);
     const ZnParams<float> zn32 = make_zn32(L->zn_n, L->zn_dv, L->zn_active);
     const int hash = p->noise_scale != 0.0 ? 1 : 0;
     if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));

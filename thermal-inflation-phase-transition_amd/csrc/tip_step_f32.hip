@@ -40,7 +40,7 @@ StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field
     StepGeometry g;
     g.by = 4;
     g.xcd = 1;
-    int64_t per_plane = ((nz + 63) / 64) * ((ny + 3) / 4);
+    int64_t per_plane = (ny * nz + 255) / 256;
     // Aim for about 4096 blocks (16 per CU) while marching at least 1 and at most 32 planes.
     int64_t cx = (nx * per_plane) / 4096;
     if (cx < 1) cx = 1;
@@ -59,27 +59,26 @@ int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz
         TIP_REQUIRE(req->cx >= 0 && req->cx <= 4096, "geometry: planes per block out of range");
         if (req->by) g.by = req->by;
         if (req->by && !req->cx) {  // re-derive the plane count for the new row count
-            int64_t per_plane = ((nz + 63) / 64) * ((ny + g.by - 1) / g.by);
+            int64_t per_plane = (ny * nz + 64 * g.by - 1) / (64 * g.by);
             int64_t cx = (nx * per_plane) / 4096;
             g.cx = (int)(cx < 1 ? 1 : (cx > 32 ? 32 : cx));
         }
         if (req->cx) g.cx = req->cx;
         g.xcd = req->xcd ? 1 : 0;
     }
-    tg.tiles_z = (int)((nz + 63) / 64);
-    tg.tiles_y = (int)((ny + g.by - 1) / g.by);
+    tg.tiles = (int)((ny * nz + 64 * g.by - 1) / (64 * g.by));
     tg.chunks = (int)((nx + g.cx - 1) / g.cx);
     tg.cx = g.cx;
-    int64_t nb = (int64_t)tg.tiles_z * tg.tiles_y * tg.chunks;
+    int64_t nb = (int64_t)tg.tiles * tg.chunks;
     TIP_REQUIRE(nb < (int64_t(1) << 31), "geometry: too many blocks");
     nblocks = (int)nb;
     tg.xcd_remap = (g.xcd && nb % 8 == 0) ? 1 : 0;
     return TIP_OK;
 }
 
-TableF32 make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd)
+SegTable make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd)
 {
-    TableF32 tab;
+    SegTable tab;
     tab.ab = reinterpret_cast<const float2*>(ab);
     tab.tmin = (float)tmin;
     tab.dinv = (float)dinv;
@@ -100,7 +99,7 @@ ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active)
 }
 
 int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi_out,
-                    const StepScalars<float>& c, const float* noise, const TableF32& tab,
+                    const StepScalars<float>& c, const float* noise, const SegTable& tab,
                     int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
                     unsigned long long* vac, float esc, const StepGeometry* geom,
                     hipStream_t st)
@@ -114,16 +113,23 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
     int nb;
     rc = resolve_geometry(geom, nx, ny, nz, 0, 0, g, tg, nb);
     if (rc) return rc;
-#define TIP_LAUNCH_REAL(BYV)                                                                   \
-    hipLaunchKernelGGL((k_rk2_real<float, TableF32, BYV>), dim3(nb), dim3(64, BYV), 0, st, phi, \
-                       pi, phi_out, pi_out, noise, c, tab, (int)nx, (int)ny, (int)nz, tg,      \
-                       noise_seed, hash, vac, esc)
-    switch (g.by) {
-        case 1: TIP_LAUNCH_REAL(1); break;
-        case 2: TIP_LAUNCH_REAL(2); break;
-        case 8: TIP_LAUNCH_REAL(8); break;
-        default: TIP_LAUNCH_REAL(4); break;
+#define TIP_LAUNCH_REAL(BYV, ODD)                                                              \
+    hipLaunchKernelGGL((k_rk2_real<float, TableF32<ODD>, BYV>), dim3(nb), dim3(64, BYV), 0, st, \
+                       phi, pi, phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny,     \
+                       (int)nz, tg, noise_seed, hash, vac, esc)
+#define TIP_LAUNCH_REAL_BY(ODD)                          \
+    switch (g.by) {                                      \
+        case 1: TIP_LAUNCH_REAL(1, ODD); break;          \
+        case 2: TIP_LAUNCH_REAL(2, ODD); break;          \
+        case 8: TIP_LAUNCH_REAL(8, ODD); break;          \
+        default: TIP_LAUNCH_REAL(4, ODD); break;         \
     }
+    if (tab.odd) {
+        TIP_LAUNCH_REAL_BY(true)
+    } else {
+        TIP_LAUNCH_REAL_BY(false)
+    }
+#undef TIP_LAUNCH_REAL_BY
 #undef TIP_LAUNCH_REAL
     TIP_HIP(hipGetLastError());
     return TIP_OK;
@@ -131,7 +137,7 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
 
 int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
                        float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
-                       const float* noise1, const float* noise2, const TableF32& tab,
+                       const float* noise1, const float* noise2, const SegTable& tab,
                        const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
                        uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
                        const StepGeometry* geom, hipStream_t st)
@@ -147,8 +153,9 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
     rc = resolve_geometry(geom, nx, ny, nz, 1, 0, g, tg, nb);
     if (rc) return rc;
 #define TIP_LAUNCH_CPLX(BYV)                                                                    \
-    hipLaunchKernelGGL((k_rk2_complex<float, TableF32, BYV>), dim3(nb), dim3(64, BYV), 0, st,    \
-                       phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, tab, zn, (int)nx, \
+    hipLaunchKernelGGL((k_rk2_complex<float, TableF32<false>, BYV>), dim3(nb), dim3(64, BYV), 0, \
+                       st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c,              \
+                       tab.as<false>(), zn, (int)nx,                                              \
                        (int)ny, (int)nz, tg, noise_seed, hash, vac, esc2)
     switch (g.by) {
         case 1: TIP_LAUNCH_CPLX(1); break;

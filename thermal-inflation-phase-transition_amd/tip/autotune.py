@@ -1,0 +1,100 @@
+"""Launch-geometry autotuner for the step kernels.
+
+The knobs are the ones the kernels expose: rows of 64 z-sites per block (1, 2, 4, 8), the
+number of x planes a block marches through, the XCD-aware block remap, and (ensembles) the
+number of persistent blocks.  Each candidate is timed with HIP events on the object's own
+stream over a few steps; the fastest is applied and remembered in a JSON cache keyed by the
+workload signature, so later runs of the same workload skip the search.
+
+Timing steps advance the object's state; tune before uploading the initial condition or
+re-upload it afterwards.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from pathlib import Path
+from typing import Dict, Iterable, List, Optional, Tuple
+
+CACHE_PATH = Path(os.environ.get("TIP_AUTOTUNE_CACHE",
+                                 Path.home() / ".cache" / "tip" / "autotune.json"))
+
+
+def _load_cache() -> Dict[str, dict]:
+    try:
+        return json.loads(CACHE_PATH.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key: str, value: dict) -> None:
+    try:
+        cache = _load_cache()
+        cache[key] = value
+        CACHE_PATH.parent.mkdir(parents=True, exist_ok=True)
+        CACHE_PATH.write_text(json.dumps(cache, indent=1, sort_keys=True))
+    except OSError:
+        pass  # a read-only home only costs the search next time
+
+
+def lattice_candidates() -> List[Tuple[int, int, int]]:
+    out = []
+    for by, cx, xcd in itertools.product((1, 2, 4, 8), (4, 8, 16, 32), (1, 0)):
+        out.append((by, cx, xcd))
+    return out
+
+
+def ensemble_candidates() -> List[Tuple[int, int, int, int]]:
+    out = []
+    for by, cx, pers in itertools.product((2, 4, 8), (4, 8, 16, 32), (0, 2048, 4096)):
+        out.append((by, cx, 1, pers))
+    return out
+
+
+def tune_lattice(lat, step_kw: dict, candidates: Optional[Iterable] = None, steps: int = 20,
+                 use_cache: bool = True) -> dict:
+    """Time each (rows, planes, xcd) geometry on a Lattice and apply the fastest."""
+    key = f"lattice:{lat.shape}:{lat.dtype.name}:{'c' if lat.complex else 'r'}"
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            lat.set_geometry(*hit["best"])
+            return hit
+    results = []
+    for cand in (candidates or lattice_candidates()):
+        lat.set_geometry(*cand)
+        lat.step(2, timed=True, **step_kw)
+        ms = lat.step(steps, timed=True, **step_kw)
+        results.append((ms / steps, tuple(cand)))
+    results.sort()
+    best = results[0][1]
+    lat.set_geometry(*best)
+    rec = {"best": list(best), "us_per_step": results[0][0] * 1e3,
+           "table": [[list(c), t * 1e3] for t, c in results]}
+    _store_cache(key, rec)
+    return rec
+
+
+def tune_ensemble(ens, candidates: Optional[Iterable] = None, steps: int = 10,
+                  mode: str = "graph", use_cache: bool = True) -> dict:
+    """Time each (rows, planes, xcd, persistent) geometry on an Ensemble and apply the best."""
+    key = f"ensemble:{sorted(set(ens.shapes))}:{len(ens.shapes)}:{mode}"
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            ens.set_geometry(*hit["best"])
+            return hit
+    results = []
+    for cand in (candidates or ensemble_candidates()):
+        ens.set_geometry(*cand)
+        ens.step(steps, mode=mode, timed=True)  # capture / warm
+        ms = ens.step(steps, mode=mode, timed=True)
+        results.append((ms / steps, tuple(cand)))
+    results.sort()
+    best = results[0][1]
+    ens.set_geometry(*best)
+    rec = {"best": list(best), "us_per_step": results[0][0] * 1e3,
+           "table": [[list(c), t * 1e3] for t, c in results]}
+    _store_cache(key, rec)
+    return rec
