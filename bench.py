@@ -163,6 +163,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, 5 device-to-device copies of 128 MiB (a known-size "
+                         "reference for the HBM byte counters of a profiler)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the "
@@ -231,6 +234,18 @@ def main() -> int:
         "true_vacuum_fraction": frac_true,
     }
     lat.close()
+    if a.copy_calibration:
+        from tip.device import DeviceArray, synchronize
+        from tip import _lib
+        nbytes = 128 << 20
+        src = DeviceArray((nbytes // 4,), np.float32)
+        dst = DeviceArray((nbytes // 4,), np.float32)
+        src.fill_zero()
+        for _ in range(5):
+            _lib.call("tip_memcpy_d2d", dst.ptr, src.ptr, nbytes, None)
+        synchronize()
+        src.free()
+        dst.free()
     if not a.no_ensemble:
         log("batched ensemble (64 realisations, 46.4M sites) ...")
         out["ensemble"] = bench_ensemble()
