@@ -276,6 +276,49 @@ int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long
 int tip_ensemble_step_count(tip_ensemble *ens, uint64_t *step);
 int tip_ensemble_set_step(tip_ensemble *ens, uint64_t step);
 
+/* ---- learned V' surrogate -----------------------------------------------------------
+ * No counterpart in the reference (which evaluates V' by host tabulation, Vprime_scalar /
+ * build_vprime_table, simulation/latticeSimeRescale_numba.py:805-849, rebuilt whenever T moves
+ * by 1e-4, :3650-3672).  A 2 -> 32 -> 32 -> 1 tanh MLP, float32, parameter vector of
+ * TIP_MLP_NPARAMS floats laid out W1[32][2], b1[32], W2[32][32], b2[32], w3[32], b3.
+ * Inputs: features x = (f1, f2), f1 = asinh(rho/rho_scale)/asinh(rho_max/rho_scale),
+ * f2 = (T - T_mid)/T_half.  V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * y * e(rho)),
+ * e = 1/(1 + (rho/rho_cut)^2): the network carries the thermal + Coleman-Weinberg part of
+ * V'/rho over a fixed envelope, the tree term is exact. */
+#define TIP_MLP_HIDDEN 32
+#define TIP_MLP_NPARAMS 1185
+
+typedef struct tip_mlp_features {
+    double rho_scale, rho_max, T_mid, T_half, out_scale, lam, m2, rho_cut;
+} tip_mlp_features;
+
+/* y[i] = MLP(x[2i], x[2i+1]) */
+int tip_mlp_forward(const float *params, const float *x, float *y, int64_t n, void *stream);
+/* gradient of L = (1/2n) sum w (y - y_target)^2 over the n samples (device float[1185]) and
+ * L (device float, nullable); weights w nullable (= 1) */
+int tip_mlp_grad(const float *params, const float *x, const float *y_target,
+                 const float *weights, int64_t n, float *grad, float *loss, void *stream);
+/* one Adam step on a minibatch (t = 1, 2, ... for the bias corrections) */
+int tip_mlp_adam_step(float *params, float *m, float *v, const float *x, const float *y_target,
+                      const float *weights, int64_t n, double lr, double beta1, double beta2,
+                      double eps, int64_t t, float *loss, void *stream);
+/* V'(rho[i], T[i]) */
+int tip_mlp_vprime(const float *params, const tip_mlp_features *fm, const float *rho,
+                   const float *T, float *vp, int64_t n, void *stream);
+/* On-device re-tabulation: table[i] = V'(x_min + i*(x_max-x_min)/(npts-1), T) (float64 out,
+ * odd in x), the table the reference would rebuild on the host. */
+int tip_mlp_tabulate(const float *params, const tip_mlp_features *fm, double T, double x_min,
+                     double x_max, int64_t npts, double *table, void *stream);
+/* Complex fused RK2 step with V'(rho, T) evaluated by the MLP at every site and stage instead
+ * of the table (the table entry point is tip_rk2_fused_c32). */
+int tip_rk2_fused_c32_mlp(const float *phi1, const float *phi2, const float *pi1,
+                          const float *pi2, float *phi1_out, float *phi2_out, float *pi1_out,
+                          float *pi2_out, double dt, double dx, double eta_eff, double mu,
+                          double inv_a2, const float *mlp_params, const tip_mlp_features *fm,
+                          double T, double zn_n, double zn_dv, double zn_active, int64_t nx,
+                          int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                          unsigned long long *vac_count, double vac_esc, void *stream);
+
 #if defined(TIP_BUILDING) && defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

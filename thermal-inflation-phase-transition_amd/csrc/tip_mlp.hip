@@ -1,0 +1,371 @@
+// tip_mlp.hip -- learned V'(rho, T) surrogate: a 2 -> 32 -> 32 -> 1 tanh MLP in float32,
+// trained on the GPU by hand-written forward/backward/Adam kernels and evaluated inline in the
+// complex-field step kernel so a continuously cooling T(t) needs no re-tabulation.
+//
+// Parameter vector (1185 float32): W1[32][2], b1[32], W2[32][32], b2[32], w3[32], b3.
+// Features: f1 = asinh(rho / rho_scale) / asinh(rho_max / rho_scale)  (resolves the thermal
+// barrier, a few T wide, and the vacuum at ~1e11 GeV in one input), f2 = (T - T_mid) / T_half.
+// Output: V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * y / (1 + (rho/rho_cut)^2)): the
+// network carries only the thermal and Coleman-Weinberg part of V'/rho (smooth,
+// Boltzmann-suppressed at large rho) divided by a fixed envelope, so its residual in the flat
+// large-rho region is suppressed; the tree term is exact.
+//
+// Training is deterministic: each 64-sample block writes its partial gradient and one kernel
+// sums the partials in block order and applies Adam.
+#include "step_kernels.h"
+
+using namespace tip;
+
+namespace {
+
+constexpr int H = TIP_MLP_HIDDEN;
+constexpr int NP = TIP_MLP_NPARAMS;
+constexpr int OW1 = 0, OB1 = 2 * H, OW2 = 3 * H, OB2 = 3 * H + H * H, OW3 = 4 * H + H * H,
+              OB3 = 5 * H + H * H;
+static_assert(OB3 + 1 == NP, "parameter layout");
+constexpr int SB = 64;  // samples per training block (one wave)
+
+__device__ __forceinline__ float tanh_f(float x)
+{
+    // tanh(x) = 1 - 2 / (exp(2x) + 1); exact limits for |x| large
+    float e = __expf(2.0f * x);
+    return 1.0f - 2.0f / (e + 1.0f);
+}
+
+struct FeatureMap {
+    float inv_rho_s, inv_norm, T_mid, inv_T_half, out_scale, inv_rho_cut;
+    double lam, m2;
+    // The tree term lam rho^3 - m2 rho cancels near the vacuum; it is formed in float64 (full
+    // rate on MI355X) so V' is rounded to float32 once, relative to its own size.
+    __device__ __forceinline__ float vprime(float rho, float y) const
+    {
+        const float q = rho * inv_rho_cut;
+        const float th = out_scale * y / fmaf(q, q, 1.0f);
+        const double r = (double)rho;
+        return (float)(r * (fma(lam * r, r, -m2) + (double)th));
+    }
+    __device__ __forceinline__ float f1(float rho) const
+    {
+        float a = rho * inv_rho_s;
+        return __logf(a + __builtin_sqrtf(a * a + 1.0f)) * inv_norm;
+    }
+    __device__ __forceinline__ float f2(float T) const { return (T - T_mid) * inv_T_half; }
+};
+
+FeatureMap make_fm(const tip_mlp_features* fm)
+{
+    FeatureMap m;
+    m.inv_rho_s = (float)(1.0 / fm->rho_scale);
+    m.inv_norm = (float)(1.0 / asinh(fm->rho_max / fm->rho_scale));
+    m.T_mid = (float)fm->T_mid;
+    m.inv_T_half = (float)(1.0 / fm->T_half);
+    m.out_scale = (float)fm->out_scale;
+    m.lam = fm->lam;
+    m.m2 = fm->m2;
+    m.inv_rho_cut = (float)(1.0 / fm->rho_cut);
+    return m;
+}
+
+int check_fm(const tip_mlp_features* fm)
+{
+    TIP_REQUIRE(fm, "mlp: NULL feature map");
+    TIP_REQUIRE(fm->rho_scale > 0 && fm->rho_max > 0 && fm->T_half > 0 && fm->rho_cut > 0,
+                "mlp: feature map scales must be positive");
+    return TIP_OK;
+}
+
+// y = MLP(f1, f2) with the weights read through uniform (scalar) loads.
+__device__ __forceinline__ float mlp_eval(const float* __restrict__ P, float x0, float x1)
+{
+    float h1[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) h1[j] = tanh_f(fmaf(P[OW1 + 2 * j], x0, fmaf(P[OW1 + 2 * j + 1], x1, P[OB1 + j])));
+    float y = P[OB3];
+#pragma unroll 4
+    for (int j = 0; j < H; ++j) {
+        float z = P[OB2 + j];
+#pragma unroll
+        for (int i = 0; i < H; ++i) z = fmaf(P[OW2 + H * j + i], h1[i], z);
+        y = fmaf(P[OW3 + j], tanh_f(z), y);
+    }
+    return y;
+}
+
+// V'(rho) at a fixed temperature -- the drop-in for the table functor in the step kernel.
+struct MlpForce {
+    const float* P;
+    FeatureMap fm;
+    float f2;  // temperature feature of this launch
+    __device__ __forceinline__ float operator()(float rho) const
+    {
+        return fm.vprime(rho, mlp_eval(P, fm.f1(rho), f2));
+    }
+};
+
+// ---- training --------------------------------------------------------------------------
+// One block = 64 samples = one wave.  Phase 1: forward/backward per sample, activations and
+// deltas staged in LDS.  Phase 2: every lane accumulates a slice of the 1185 parameter
+// gradients over the block's samples (fixed order).
+__global__ __launch_bounds__(SB) void k_mlp_fwd_bwd(const float* __restrict__ P,
+                                                    const float2* __restrict__ X,
+                                                    const float* __restrict__ Y,
+                                                    const float* __restrict__ Wt, int n,
+                                                    float inv_n, float* __restrict__ part,
+                                                    float* __restrict__ loss_part)
+{
+    __shared__ float sh1[SB][H + 1], sh2[SB][H + 1], sd2[SB][H + 1], sd1[SB][H + 1];
+    __shared__ float sx[SB][2], sdy[SB], sloss[SB];
+    const int t = threadIdx.x;
+    const int s = blockIdx.x * SB + t;
+    const bool act = s < n;
+    float2 x = act ? X[s] : make_float2(0.f, 0.f);
+    float h1[H], h2[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+        h1[j] = tanh_f(fmaf(P[OW1 + 2 * j], x.x, fmaf(P[OW1 + 2 * j + 1], x.y, P[OB1 + j])));
+    float y = P[OB3];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        float z = P[OB2 + j];
+#pragma unroll
+        for (int i = 0; i < H; ++i) z = fmaf(P[OW2 + H * j + i], h1[i], z);
+        h2[j] = tanh_f(z);
+        y = fmaf(P[OW3 + j], h2[j], y);
+    }
+    const float r = act ? (y - Y[s]) : 0.0f;
+    const float w = (act && Wt) ? Wt[s] : 1.0f;
+    const float dy = w * r * inv_n;  // d(0.5 mean w r^2)/dy
+    sloss[t] = 0.5f * w * r * r;
+    sdy[t] = dy;
+    sx[t][0] = act ? x.x : 0.f;
+    sx[t][1] = act ? x.y : 0.f;
+    float d2[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        d2[j] = dy * P[OW3 + j] * (1.0f - h2[j] * h2[j]);
+        sh1[t][j] = act ? h1[j] : 0.f;
+        sh2[t][j] = act ? h2[j] : 0.f;
+        sd2[t][j] = act ? d2[j] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        float g = 0.f;
+#pragma unroll
+        for (int j = 0; j < H; ++j) g = fmaf(P[OW2 + H * j + i], d2[j], g);
+        sd1[t][i] = act ? g * (1.0f - h1[i] * h1[i]) : 0.f;
+    }
+    __syncthreads();
+    float* out = part + (size_t)blockIdx.x * NP;
+    for (int p = t; p < NP; p += SB) {
+        float g = 0.f;
+        if (p < OB1) {  // W1[j][k]
+            const int j = p >> 1, k = p & 1;
+            for (int q = 0; q < SB; ++q) g = fmaf(sd1[q][j], sx[q][k], g);
+        } else if (p < OW2) {  // b1[j]
+            const int j = p - OB1;
+            for (int q = 0; q < SB; ++q) g += sd1[q][j];
+        } else if (p < OB2) {  // W2[j][i]
+            const int j = (p - OW2) / H, i = (p - OW2) % H;
+            for (int q = 0; q < SB; ++q) g = fmaf(sd2[q][j], sh1[q][i], g);
+        } else if (p < OW3) {  // b2[j]
+            const int j = p - OB2;
+            for (int q = 0; q < SB; ++q) g += sd2[q][j];
+        } else if (p < OB3) {  // w3[j]
+            const int j = p - OW3;
+            for (int q = 0; q < SB; ++q) g = fmaf(sdy[q], sh2[q][j], g);
+        } else {  // b3
+            for (int q = 0; q < SB; ++q) g += sdy[q];
+        }
+        out[p] = g;
+    }
+    if (t == 0) {
+        float l = 0.f;
+        for (int q = 0; q < SB; ++q) l += sloss[q];
+        loss_part[blockIdx.x] = l;
+    }
+}
+
+// Sum the per-block partials (block order) into the gradient; optionally apply Adam.
+__global__ void k_mlp_reduce(const float* __restrict__ part, const float* __restrict__ loss_part,
+                             int nblocks, float* __restrict__ grad, float* __restrict__ loss,
+                             float inv_n, float* __restrict__ P, float* __restrict__ m,
+                             float* __restrict__ v, float lr, float b1, float b2, float eps,
+                             float bc1, float bc2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && loss) {
+        double l = 0.0;
+        for (int b = 0; b < nblocks; ++b) l += loss_part[b];
+        loss[0] = (float)(l * inv_n);
+    }
+    if (p >= NP) return;
+    float g = 0.f;
+    for (int b = 0; b < nblocks; ++b) g += part[(size_t)b * NP + p];
+    if (grad) grad[p] = g;
+    if (P) {
+        const float mm = b1 * m[p] + (1.0f - b1) * g;
+        const float vv = b2 * v[p] + (1.0f - b2) * g * g;
+        m[p] = mm;
+        v[p] = vv;
+        P[p] -= lr * (mm / bc1) / (__builtin_sqrtf(vv / bc2) + eps);
+    }
+}
+
+__global__ void k_mlp_forward(const float* __restrict__ P, const float2* __restrict__ X,
+                              float* __restrict__ Y, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float2 x = X[s];
+    Y[s] = mlp_eval(P, x.x, x.y);
+}
+
+__global__ void k_mlp_vprime(const float* __restrict__ P, FeatureMap fm,
+                             const float* __restrict__ rho, const float* __restrict__ T,
+                             float* __restrict__ vp, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float r = rho[s];
+    vp[s] = fm.vprime(r, mlp_eval(P, fm.f1(r), fm.f2(T[s])));
+}
+
+__global__ void k_mlp_tabulate(const float* __restrict__ P, FeatureMap fm, float f2, double x0,
+                               double h, int n, double* __restrict__ table)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float r = (float)(x0 + h * (double)i);
+    const float a = fabsf(r);
+    const float v = fm.vprime(a, mlp_eval(P, fm.f1(a), f2));
+    table[i] = (double)(r < 0.0f ? -v : v);
+}
+
+int train_pass(const float* P, const float* X, const float* Y, const float* Wt, int64_t n,
+               float* grad,
+               float* loss, float* Pw, float* m, float* v, double lr, double b1, double b2,
+               double eps, int64_t t, hipStream_t st)
+{
+    TIP_REQUIRE(P && X && Y, "mlp: NULL parameters or data");
+    TIP_REQUIRE(n >= 1 && n < (int64_t(1) << 30), "mlp: batch size out of range");
+    const int nb = (int)((n + SB - 1) / SB);
+    float* part = reinterpret_cast<float*>(scratch_get(((size_t)nb * NP + nb) * sizeof(float), 2));
+    TIP_REQUIRE(part, "mlp: scratch allocation failed");
+    float* lpart = part + (size_t)nb * NP;
+    const float inv_n = (float)(1.0 / (double)n);
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nb), dim3(SB), 0, st, P,
+                       reinterpret_cast<const float2*>(X), Y, Wt, (int)n, inv_n, part, lpart);
+    TIP_HIP(hipGetLastError());
+    float bc1 = 1.f, bc2 = 1.f;
+    if (Pw) {
+        TIP_REQUIRE(m && v && t >= 1, "mlp adam: moments and step t >= 1 are required");
+        bc1 = (float)(1.0 - pow(b1, (double)t));
+        bc2 = (float)(1.0 - pow(b2, (double)t));
+    }
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((NP + 255) / 256), dim3(256), 0, st, part, lpart, nb,
+                       grad, loss, inv_n, Pw, m, v, (float)lr, (float)b1,
+                       (float)b2, (float)eps, bc1, bc2);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_mlp_forward(const float* params, const float* x, float* y, int64_t n,
+                               void* stream)
+{
+    TIP_REQUIRE(params && x && y && n >= 0 && n < (int64_t(1) << 31), "tip_mlp_forward: bad args");
+    if (n == 0) return TIP_OK;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), params, reinterpret_cast<const float2*>(x), y, (int)n);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+extern "C" int tip_mlp_grad(const float* params, const float* x, const float* y_target,
+                            const float* weights, int64_t n, float* grad, float* loss,
+                            void* stream)
+{
+    TIP_REQUIRE(grad, "tip_mlp_grad: NULL grad");
+    return train_pass(params, x, y_target, weights, n, grad, loss, nullptr, nullptr, nullptr, 0,
+                      0, 0, 0, 0, as_stream(stream));
+}
+
+extern "C" int tip_mlp_adam_step(float* params, float* m, float* v, const float* x,
+                                 const float* y_target, const float* weights, int64_t n,
+                                 double lr, double beta1, double beta2, double eps, int64_t t,
+                                 float* loss, void* stream)
+{
+    return train_pass(params, x, y_target, weights, n, nullptr, loss, params, m, v, lr, beta1,
+                      beta2, eps, t, as_stream(stream));
+}
+
+extern "C" int tip_mlp_vprime(const float* params, const tip_mlp_features* fm, const float* rho,
+                              const float* T, float* vp, int64_t n, void* stream)
+{
+    int rc = check_fm(fm);
+    if (rc) return rc;
+    TIP_REQUIRE(params && rho && T && vp && n >= 0 && n < (int64_t(1) << 31),
+                "tip_mlp_vprime: bad args");
+    if (n == 0) return TIP_OK;
+    hipLaunchKernelGGL(k_mlp_vprime, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), params, make_fm(fm), rho, T, vp, (int)n);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+extern "C" int tip_mlp_tabulate(const float* params, const tip_mlp_features* fm, double T,
+                                double x_min, double x_max, int64_t npts, double* table,
+                                void* stream)
+{
+    int rc = check_fm(fm);
+    if (rc) return rc;
+    TIP_REQUIRE(params && table && npts >= 2 && npts < (int64_t(1) << 30) && x_max > x_min,
+                "tip_mlp_tabulate: bad args");
+    FeatureMap m = make_fm(fm);
+    hipLaunchKernelGGL(k_mlp_tabulate, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), params, m, (float)((T - fm->T_mid) / fm->T_half), x_min,
+                       (x_max - x_min) / (double)(npts - 1), (int)npts, table);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+extern "C" int tip_rk2_fused_c32_mlp(const float* phi1, const float* phi2, const float* pi1,
+                                     const float* pi2, float* phi1_out, float* phi2_out,
+                                     float* pi1_out, float* pi2_out, double dt, double dx,
+                                     double eta_eff, double mu, double inv_a2,
+                                     const float* mlp_params, const tip_mlp_features* fm,
+                                     double T, double zn_n, double zn_dv, double zn_active,
+                                     int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
+                                     double noise_scale, unsigned long long* vac_count,
+                                     double vac_esc, void* stream)
+{
+    int rc = check_fm(fm);
+    if (rc) return rc;
+    rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && phi1_out && phi2_out && pi1_out && pi2_out &&
+                    mlp_params,
+                "rk2 c32 mlp: NULL pointer");
+    TIP_REQUIRE(phi1 != phi1_out && phi2 != phi2_out && pi1 != pi1_out && pi2 != pi2_out,
+                "rk2 c32 mlp: output must not alias input");
+    MlpForce f;
+    f.P = mlp_params;
+    f.fm = make_fm(fm);
+    f.f2 = (float)((T - fm->T_mid) / fm->T_half);
+    StepGeometry g;
+    TileGrid tg;
+    int nb;
+    StepGeometry req = {1, 0, 1};
+    rc = resolve_geometry(&req, nx, ny, nz, 1, 0, g, tg, nb);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_rk2_complex<float, MlpForce, 1>), dim3(nb), dim3(64, 1), 0,
+                       as_stream(stream), phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out,
+                       pi2_out, nullptr, nullptr,
+                       make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), f,
+                       make_zn32(zn_n, zn_dv, zn_active), (int)nx, (int)ny, (int)nz, tg,
+                       noise_seed, noise_scale != 0.0 ? 1 : 0, vac_count,
+                       (float)(vac_esc * vac_esc));
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}

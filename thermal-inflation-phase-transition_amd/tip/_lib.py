@@ -30,6 +30,15 @@ class StepParams(C.Structure):
 
 _ppv = C.POINTER(C.c_void_p)
 
+
+class MlpFeatures(C.Structure):
+    """tip_mlp_features of include/tip.h."""
+
+    _fields_ = [("rho_scale", C.c_double), ("rho_max", C.c_double), ("T_mid", C.c_double),
+                ("T_half", C.c_double), ("out_scale", C.c_double), ("lam", C.c_double),
+                ("m2", C.c_double), ("rho_cut", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol of include/tip.h
 SIGNATURES = {
     "tip_version": (_int, [_pint, _pint]),
@@ -105,6 +114,15 @@ SIGNATURES = {
     "tip_ensemble_vacuum_counts": (_int, [_vp, _d, _vp]),
     "tip_ensemble_step_count": (_int, [_vp, C.POINTER(C.c_uint64)]),
     "tip_ensemble_set_step": (_int, [_vp, _u64]),
+    "tip_mlp_forward": (_int, [_vp, _vp, _vp, _i64, _vp]),
+    "tip_mlp_grad": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "tip_mlp_adam_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _d, _i64, _vp,
+                                 _vp]),
+    "tip_mlp_vprime": (_int, [_vp, C.POINTER(MlpFeatures), _vp, _vp, _vp, _i64, _vp]),
+    "tip_mlp_tabulate": (_int, [_vp, C.POINTER(MlpFeatures), _d, _d, _d, _i64, _vp, _vp]),
+    "tip_rk2_fused_c32_mlp": (_int, [
+        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp,
+        C.POINTER(MlpFeatures), _d, _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp, _d, _vp]),
 }
 
 
