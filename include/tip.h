@@ -319,6 +319,14 @@ int tip_rk2_fused_c32_mlp(const float *phi1, const float *phi2, const float *pi1
                           int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
                           unsigned long long *vac_count, double vac_esc, void *stream);
 
+/* nsteps complex-field steps of a device-resident lattice with V'(rho, T) from the surrogate,
+ * T = T_start + dT_per_step * n at step n (continuous cooling, no re-tabulation).  The noise
+ * amplitude p->noise_scale is the one at T_start; step n uses noise_scale * sqrt(T_n / T_start),
+ * the T dependence of sqrt(2 eta T dt / (mu^2 dx^3)).  T must stay positive. */
+int tip_lattice_step_mlp(tip_lattice *lat, const tip_step_params *p, const float *mlp_params,
+                         const tip_mlp_features *fm, double T_start, double dT_per_step,
+                         int64_t nsteps, uint64_t step0, uint64_t seed, float *elapsed_ms);
+
 #if defined(TIP_BUILDING) && defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

@@ -1,0 +1,85 @@
+"""The lattice driver on the GPU (-m gpu): its main loop replays exactly on the oracle, and the
+surrogate path's multi-step calls equal one call per step.
+
+Bars:
+  * real float64 with Hubble cooling, 30 steps at 32^3, tables rebuilt by the driver's rule:
+    fields within 1e-12 relative of the oracle fed the same tables, schedule and noise seeds;
+    true-vacuum series exact
+  * complex float32 with the surrogate under linear cooling: one call per report interval
+    (noise amplitude scaled by sqrt(T_n / T_start) inside) within 1e-5 relative of one call per
+    step (noise amplitude from the formula at every step)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_driver_real_f64_replays_on_oracle():
+    from tip import driver as D
+    from tip import potential as P
+    cfg = D.RunConfig(field="real", Nx=32, Ny=32, Nz=32, Nt=30, T0=7000.0, precision="f64",
+                      no_hubble=False, report=10, seed=3, quiet=True, table_size=4096)
+    run = D.Run(cfg)
+    tables = []  # (step index the table is first used at, table, tmin, dinv)
+    orig = run.lat.set_table
+
+    def rec(table, tmin, dinv, odd=False):
+        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
+        orig(table, tmin, dinv, odd=odd)
+
+    run.lat.set_table = rec
+    try:
+        series = run.run()
+        phi_g, pi_g = run.lat.download("phi"), run.lat.download("pi")
+    finally:
+        run.close()
+    assert len(tables) >= 1 and series["step"] == [10, 20, 30]
+
+    f0 = D.initial_fields(cfg)
+    phi, pi = f0["phi"].copy(), f0["pi"].copy()
+    pt, qt = np.empty_like(phi), np.empty_like(pi)
+    hub = P.HubbleState(cfg.T0, D.MU, cfg.eta_phys / D.MU)
+    dt, dx = D.MU * cfg.dt_phys, D.MU * cfg.dx_phys
+    fr = []
+    k = 0
+    for n in range(cfg.Nt):
+        while k + 1 < len(tables) and tables[k + 1][0] <= n:
+            k += 1
+        _, t, tmin, dinv = tables[k]
+        s = hub.step(dt)
+        ns = math.sqrt(2.0 * s["eta_eff"] * s["T"] * dt * s["inv_a3"] / (D.MU ** 2 * cfg.dx_phys ** 3))
+        noise = O.noise_field(phi.shape, O.noise_seed(n, cfg.seed), ns)
+        O.rk2_step_table(phi, pi, dt, dx, s["eta_eff"], D.MU, noise, pt, qt, s["inv_a2"], t,
+                         tmin, dinv, t.size)
+        if (n + 1) % cfg.report == 0:
+            fr.append(O.vacuum_count(phi, cfg.esc) / phi.size)
+    assert np.linalg.norm(phi_g - phi) <= 1e-12 * np.linalg.norm(phi)
+    assert np.linalg.norm(pi_g - pi) <= 1e-12 * np.linalg.norm(pi)
+    assert series["f_true"] == fr
+
+
+def test_surrogate_chunked_equals_stepwise():
+    from tip import driver as D
+    from tip import surrogate as S
+    sur = S.Surrogate(S.SurrogateSpec(), params=S.init_params(11))
+    out = []
+    for chunked in (True, False):
+        cfg = D.RunConfig(field="complex", Nx=32, Ny=32, Nz=64, Nt=40, T0=7350.0,
+                          cool_to=6770.0, report=20, seed=5, quiet=True, surrogate=sur,
+                          init_rho=50.0)
+        run = D.Run(cfg)
+        if not chunked:
+            run._chunk = lambda n: 1
+        try:
+            run.run()
+            out.append([run.lat.download(k).astype(np.float64)
+                        for k in ("phi1", "phi2", "pi1", "pi2")])
+        finally:
+            run.close()
+    for a, b in zip(*out):
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)

@@ -391,6 +391,13 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
                        uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
                        const StepGeometry* geom, hipStream_t st);
 SegTable make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd);
+// Complex fused step with the V' surrogate evaluated inline (tip_mlp.hip).
+int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
+                       float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
+                       const float* mlp_params, const tip_mlp_features* fm, double T,
+                       const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
+                       uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
+                       hipStream_t st);
 ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active);
 
 }  // namespace tip

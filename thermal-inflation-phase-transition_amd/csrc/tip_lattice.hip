@@ -235,6 +235,42 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
     return TIP_OK;
 }
 
+extern "C" int tip_lattice_step_mlp(tip_lattice* L, const tip_step_params* p,
+                                    const float* mlp_params, const tip_mlp_features* fm, double T,
+                                    double dT_per_step, int64_t nsteps, uint64_t step0,
+                                    uint64_t seed, float* elapsed_ms)
+{
+    TIP_REQUIRE(L && p && mlp_params && fm, "tip_lattice_step_mlp: NULL");
+    TIP_REQUIRE(L->kind == TIP_COMPLEX, "tip_lattice_step_mlp: needs a complex lattice");
+    TIP_REQUIRE(nsteps >= 0, "tip_lattice_step_mlp: nsteps < 0");
+    TIP_REQUIRE(T > 0.0 && T + dT_per_step * (double)nsteps > 0.0,
+                "tip_lattice_step_mlp: temperature must stay positive");
+    const ZnParams<float> zn32 = make_zn32(L->zn_n, L->zn_dv, L->zn_active);
+    const int hash = p->noise_scale != 0.0 ? 1 : 0;
+    if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        void** in = L->buf[L->cur];
+        void** out = L->buf[L->cur ^ 1];
+        // The Langevin amplitude sqrt(2 eta T dt / (mu^2 dx^3)) follows the cooling T.
+        const double Tn = T + dT_per_step * (double)n;
+        const StepScalars<float> c32 = make_scalars<float>(
+            p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale * sqrt(Tn / T));
+        int rc = launch_complex_mlp(
+            (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3],
+            (float*)out[0], (float*)out[1], (float*)out[2], (float*)out[3], c32, mlp_params, fm,
+            Tn, zn32, L->nx, L->ny, L->nz,
+            tip_noise_seed(step0 + (uint64_t)n, seed), hash, nullptr, 0.0f, L->stream);
+        if (rc) return rc;
+        L->cur ^= 1;
+    }
+    if (elapsed_ms) {
+        TIP_HIP(hipEventRecord(L->ev1, L->stream));
+        TIP_HIP(hipEventSynchronize(L->ev1));
+        TIP_HIP(hipEventElapsedTime(elapsed_ms, L->ev0, L->ev1));
+    }
+    return TIP_OK;
+}
+
 extern "C" int tip_lattice_vacuum_count(tip_lattice* L, double esc, unsigned long long* count)
 {
     TIP_REQUIRE(L && count, "tip_lattice_vacuum_count: NULL");

@@ -330,24 +330,22 @@ extern "C" int tip_mlp_tabulate(const float* params, const tip_mlp_features* fm,
     return TIP_OK;
 }
 
-extern "C" int tip_rk2_fused_c32_mlp(const float* phi1, const float* phi2, const float* pi1,
-                                     const float* pi2, float* phi1_out, float* phi2_out,
-                                     float* pi1_out, float* pi2_out, double dt, double dx,
-                                     double eta_eff, double mu, double inv_a2,
-                                     const float* mlp_params, const tip_mlp_features* fm,
-                                     double T, double zn_n, double zn_dv, double zn_active,
-                                     int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
-                                     double noise_scale, unsigned long long* vac_count,
-                                     double vac_esc, void* stream)
+namespace tip {
+
+int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
+                       float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,
+                       const float* mlp_params, const tip_mlp_features* fm, double T,
+                       const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
+                       uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
+                       hipStream_t st)
 {
     int rc = check_fm(fm);
     if (rc) return rc;
     rc = check_dims(nx, ny, nz);
     if (rc) return rc;
-    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && phi1_out && phi2_out && pi1_out && pi2_out &&
-                    mlp_params,
+    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && mlp_params,
                 "rk2 c32 mlp: NULL pointer");
-    TIP_REQUIRE(phi1 != phi1_out && phi2 != phi2_out && pi1 != pi1_out && pi2 != pi2_out,
+    TIP_REQUIRE(phi1 != o1 && phi2 != o2 && pi1 != o3 && pi2 != o4,
                 "rk2 c32 mlp: output must not alias input");
     MlpForce f;
     f.P = mlp_params;
@@ -359,13 +357,28 @@ extern "C" int tip_rk2_fused_c32_mlp(const float* phi1, const float* phi2, const
     StepGeometry req = {1, 0, 1};
     rc = resolve_geometry(&req, nx, ny, nz, 1, 0, g, tg, nb);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_rk2_complex<float, MlpForce, 1>), dim3(nb), dim3(64, 1), 0,
-                       as_stream(stream), phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out,
-                       pi2_out, nullptr, nullptr,
-                       make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), f,
-                       make_zn32(zn_n, zn_dv, zn_active), (int)nx, (int)ny, (int)nz, tg,
-                       noise_seed, noise_scale != 0.0 ? 1 : 0, vac_count,
-                       (float)(vac_esc * vac_esc));
+    hipLaunchKernelGGL((k_rk2_complex<float, MlpForce, 1>), dim3(nb), dim3(64, 1), 0, st, phi1,
+                       phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, f, zn, (int)nx, (int)ny,
+                       (int)nz, tg, noise_seed, hash, vac, esc2);
     TIP_HIP(hipGetLastError());
     return TIP_OK;
+}
+
+}  // namespace tip
+
+extern "C" int tip_rk2_fused_c32_mlp(const float* phi1, const float* phi2, const float* pi1,
+                                     const float* pi2, float* phi1_out, float* phi2_out,
+                                     float* pi1_out, float* pi2_out, double dt, double dx,
+                                     double eta_eff, double mu, double inv_a2,
+                                     const float* mlp_params, const tip_mlp_features* fm,
+                                     double T, double zn_n, double zn_dv, double zn_active,
+                                     int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
+                                     double noise_scale, unsigned long long* vac_count,
+                                     double vac_esc, void* stream)
+{
+    return launch_complex_mlp(phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out, pi2_out,
+                              make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale),
+                              mlp_params, fm, T, make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz,
+                              noise_seed, noise_scale != 0.0 ? 1 : 0, vac_count,
+                              (float)(vac_esc * vac_esc), as_stream(stream));
 }

@@ -1,0 +1,108 @@
+"""Config-level experiments on top of the driver: the ensemble nucleation run (config 3) and
+the surrogate-versus-table comparison under continuous cooling (config 4).
+
+  python -m tip.experiments surrogate-vs-table --n 64 --steps 4000 --seeds 4
+  python -m tip.experiments nucleation --steps 20000
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from typing import Dict, Sequence
+
+import numpy as np
+
+from .driver import Run, RunConfig, nucleation_ensemble
+
+
+def surrogate_vs_table(surrogate, n: int = 64, steps: int = 4000, T0: float = 7350.0,
+                       T1: float = 6770.0, seeds: Sequence[int] = (0, 1, 2, 3),
+                       report: int = 250, esc: float = 1e4, log=None) -> Dict:
+    """Complex field, linear cooling T0 -> T1 over `steps`, an ensemble of seeds run twice:
+    with the reference's table method (host-built 16384-point table on the field range,
+    rebuilt when |dT|/T > 1e-4 or the field leaves the range) and with the inline surrogate
+    at the exact T(t).  Returns per-report-step ensemble means and standard deviations of the
+    true-vacuum fraction and the string count, and whether the two agree within 1 sigma
+    (|mean_s - mean_t| <= sqrt(sd_s^2 + sd_t^2) / sqrt(len(seeds)) ... reported, not imposed
+    here)."""
+    out = {}
+    for method in ("table", "surrogate"):
+        F, S, el, rebuilds = [], [], 0.0, 0
+        for sd in seeds:
+            cfg = RunConfig(field="complex", Nx=n, Ny=n, Nz=n, Nt=steps, T0=T0, cool_to=T1,
+                            seed=sd, report=report, esc=esc, quiet=True,
+                            surrogate=surrogate if method == "surrogate" else None,
+                            divergence=0.0)
+            run = Run(cfg)
+            try:
+                ser = run.run()
+            finally:
+                run.close()
+            F.append(ser["f_true"])
+            S.append(ser["strings"])
+            el += run.elapsed
+            rebuilds += run.rebuilds
+            if log:
+                log(f"{method} seed {sd}: {run.elapsed:.1f}s, final f_true {ser['f_true'][-1]:.4f}, "
+                    f"strings {ser['strings'][-1]}")
+        F, S = np.array(F, dtype=np.float64), np.array(S, dtype=np.float64)
+        out[method] = {"f_mean": F.mean(0), "f_sd": F.std(0, ddof=1) if len(seeds) > 1 else 0 * F[0],
+                       "s_mean": S.mean(0), "s_sd": S.std(0, ddof=1) if len(seeds) > 1 else 0 * S[0],
+                       "elapsed_s": el, "ms_per_step": 1e3 * el / (steps * len(seeds)),
+                       "table_rebuilds": rebuilds}
+        out["steps_axis"] = list(range(report, steps + 1, report))
+    t, s = out["table"], out["surrogate"]
+
+    def within(m1, s1, m2, s2):
+        tol = np.sqrt(s1 ** 2 + s2 ** 2)
+        return np.abs(m1 - m2) <= np.maximum(tol, 1e-12)
+
+    out["f_within_1sigma"] = within(t["f_mean"], t["f_sd"], s["f_mean"], s["f_sd"])
+    out["s_within_1sigma"] = within(t["s_mean"], t["s_sd"], s["s_mean"], s["s_sd"])
+    return out
+
+
+def _jsonable(o):
+    if isinstance(o, dict):
+        return {k: _jsonable(v) for k, v in o.items()}
+    if isinstance(o, (np.ndarray, list, tuple)):
+        return [_jsonable(v) for v in o]
+    if isinstance(o, (np.floating, np.integer, np.bool_)):
+        return o.item()
+    return o
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    a1 = sub.add_parser("surrogate-vs-table")
+    a1.add_argument("--surrogate", default=None, help="trained npz (default: train now)")
+    a1.add_argument("--n", type=int, default=64)
+    a1.add_argument("--steps", type=int, default=4000)
+    a1.add_argument("--seeds", type=int, default=4)
+    a1.add_argument("--report", type=int, default=250)
+    a2 = sub.add_parser("nucleation")
+    a2.add_argument("--steps", type=int, default=20000)
+    a2.add_argument("--report", type=int, default=500)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.what == "surrogate-vs-table":
+        from .surrogate import Surrogate, train_config4
+        sur = Surrogate.load(a.surrogate) if a.surrogate else train_config4(log=log)[0]
+        res = surrogate_vs_table(sur, n=a.n, steps=a.steps, seeds=range(a.seeds),
+                                 report=a.report, log=log)
+    else:
+        res = nucleation_ensemble(steps=a.steps, report=a.report, log=log)
+    res = _jsonable(res)
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(res, fh)
+    print(json.dumps({k: v for k, v in res.items() if k not in ("fractions",)})[:4000])
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

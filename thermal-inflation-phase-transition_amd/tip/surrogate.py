@@ -20,6 +20,8 @@ Accuracy metric: |dV'| / max(|V'|, 1e-3 T^4/(2 pi^2) n) as in tools/validate_lan
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
+import json
 import math
 from dataclasses import dataclass, field
 from typing import Dict, Optional
@@ -216,9 +218,22 @@ class Surrogate:
     def numpy_params(self) -> np.ndarray:
         return self.params.download()
 
-    def save(self, path) -> None:
-        np.savez(path, params=self.numpy_params(), rho_scale=self.spec.rho_scale,
-                 rho_max=self.spec.rho_max, T_lo=self.spec.T_lo, T_hi=self.spec.T_hi)
+    def save(self, path, **meta) -> None:
+        sp = self.spec
+        np.savez(path, params=self.numpy_params(), rho_scale=sp.rho_scale, rho_cut=sp.rho_cut,
+                 rho_frac=sp.rho_frac, T_lo=sp.T_lo, T_hi=sp.T_hi, T_ref=sp.T_ref,
+                 model=json.dumps(dataclasses.asdict(sp.model)), include_cw=sp.include_cw,
+                 meta=json.dumps(meta))
+
+    @classmethod
+    def load(cls, path) -> "Surrogate":
+        d = np.load(path, allow_pickle=False)
+        model = P.ModelParams(**json.loads(str(d["model"])))
+        spec = SurrogateSpec(model=model, T_lo=float(d["T_lo"]), T_hi=float(d["T_hi"]),
+                             T_ref=float(d["T_ref"]), rho_frac=float(d["rho_frac"]),
+                             rho_scale=float(d["rho_scale"]), rho_cut=float(d["rho_cut"]),
+                             include_cw=bool(d["include_cw"]))
+        return cls(spec, params=d["params"])
 
     def error_report(self, n: int = 1 << 16, seed: int = 99) -> Dict[str, float]:
         """Floored relative error of V' against the float64 truth on fresh samples (float32
@@ -236,6 +251,17 @@ class Surrogate:
                 "median_rel_floored": float(np.median(rel)),
                 "max_rel_barrier_region": float(rel_low.max()) if low.any() else 0.0,
                 "n": int(n)}
+
+
+def train_config4(epochs: int = 400, n_samples: int = 1 << 20, seed: int = 0, log=None):
+    """The config-4 recipe: complex-field potential, T in [6500, 7500], 2^20 float64-truth
+    samples (5 % T margin), weighted loss, GPU Adam 400 epochs, batch 1024, cosine lr
+    2e-3 -> 1e-6."""
+    spec = SurrogateSpec()
+    rho, T = spec.sample(n_samples, seed=seed, T_margin=0.05)
+    sur = Surrogate(spec, seed=seed)
+    hist = sur.train(rho, T, epochs=epochs, batch=1024, lr=2e-3, lr_end=1e-6, log=log)
+    return sur, hist
 
 
 def table_error(spec: SurrogateSpec, table, tmin, dinv, T: float, n: int = 1 << 14,
