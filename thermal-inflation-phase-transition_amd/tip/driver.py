@@ -61,6 +61,7 @@ class RunConfig:
     zn_strength: float = 0.0
     zn_turn_on_T: float = 0.0
     init_rho: float = 0.01
+    counterterm: bool = False  # lattice counterterm on the dJ splines (:458-490, :758-795)
     precision: str = "f32"
     seed: int = 0
     table_size: int = 16384
@@ -120,6 +121,8 @@ class Run:
         self.dx = MU * cfg.dx_phys
         self.eta = cfg.eta_phys / MU
         self.hubble = None if cfg.no_hubble else P.HubbleState(cfg.T0, MU, self.eta)
+        self.spline = (P.dj_spline(x_cut=P.counterterm_xcut(cfg.dx_phys, cfg.T0))
+                       if cfg.counterterm else P.dj_spline())
         self.table_T = None
         self.table_lo = self.table_hi = None
         self.rebuilds = 0
@@ -168,7 +171,8 @@ class Run:
         margin = max(rng * TABLE_MARGIN_FRAC, 20000.0)
         t_lo = max(0.0, lo - margin) if self.cplx else lo - margin
         t_hi = hi + margin
-        table, tmin, dinv = P.build_vprime_table(T, t_lo, t_hi, self.cfg.table_size, self.model)
+        table, tmin, dinv = P.build_vprime_table(T, t_lo, t_hi, self.cfg.table_size, self.model,
+                                                 self.spline)
         self.lat.set_table(table, tmin, dinv, odd=False)
         self.table_T, self.table_lo, self.table_hi = T, t_lo, t_hi
         self.rebuilds += 1

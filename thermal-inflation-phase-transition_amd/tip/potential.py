@@ -81,6 +81,18 @@ def dJf(u) -> np.ndarray:
     return _bessel_sum(u, 1, True, 1)
 
 
+def d2J(u, fermion: bool) -> np.ndarray:
+    """d^2 J/du^2 = dJ/du / u - sum_n s_n u^2 K_0(n u) (from d/dx[x K_1(x)] = -x K_0(x));
+    pi^2/6 (boson) or pi^2/12 (fermion) at u = 0."""
+    u = np.atleast_1d(np.asarray(u, dtype=np.float64))
+    out = np.empty_like(u)
+    pos = u > 0
+    d1 = dJf(u[pos]) if fermion else dJb(u[pos])
+    out[pos] = d1 / u[pos] - _bessel_sum(u[pos], 0, fermion, 0)
+    out[~pos] = math.pi ** 2 / (12.0 if fermion else 6.0)
+    return out
+
+
 def Jb(u) -> np.ndarray:
     u = np.atleast_1d(np.asarray(u, dtype=np.float64))
     out = -_bessel_sum(u, 2, False, 2)
@@ -114,12 +126,45 @@ class DJSpline:
 _SPLINE_CACHE: dict = {}
 
 
-def dj_spline(n_y: int = N_Y, ymax: float = YMAX) -> DJSpline:
-    key = (n_y, ymax)
+def dJ_soft(u, x_cut: float, fermion: bool) -> np.ndarray:
+    """Soft-mode (k < pi/dx) part of dJ/du as the reference's lattice counterterm subtracts it
+    (latticeSimeRescale_numba.py:460-490): (1/2pi^2) int_0^x_cut x^2 u / (z (e^z -+ 1)) dx,
+    z = sqrt(x^2 + u^2), integrand cut to 0 for z > 50; 0 for x_cut < 1e-10 or u < 1e-30."""
+    from scipy.integrate import quad
+    u = np.atleast_1d(np.asarray(u, dtype=np.float64))
+    out = np.zeros_like(u)
+    if x_cut < 1e-10:
+        return out
+    sgn = 1.0 if fermion else -1.0
+    for i, y in enumerate(u):
+        if y < 1e-30:
+            continue
+
+        def f(x, y=y):
+            z = math.sqrt(x * x + y * y)
+            return 0.0 if z > 50.0 else x * x * y / (z * (math.exp(z) + sgn))
+
+        out[i] = quad(f, 0.0, x_cut, limit=200)[0] / (2.0 * math.pi ** 2)
+    return out
+
+
+def counterterm_xcut(dx_phys: float, T0: float) -> float:
+    """x_cut = k_max / T0 with k_max = pi / dx_phys (latticeSimeRescale_numba.py:759)."""
+    return math.pi / (dx_phys * T0)
+
+
+def dj_spline(n_y: int = N_Y, ymax: float = YMAX, x_cut: float = 0.0) -> DJSpline:
+    """The dJ_b, dJ_f splines; x_cut > 0 applies the lattice counterterm (the soft-mode part
+    subtracted from the grid values before the spline, :758-795)."""
+    key = (n_y, ymax, x_cut)
     if key not in _SPLINE_CACHE:
         grid = np.linspace(0.0, ymax, n_y, dtype=np.float64)
-        cs_b = CubicSpline(grid, dJb(grid), bc_type="not-a-knot")
-        cs_f = CubicSpline(grid, dJf(grid), bc_type="not-a-knot")
+        gb, gf = dJb(grid), dJf(grid)
+        if x_cut > 0.0:
+            gb = gb - dJ_soft(grid, x_cut, fermion=False)
+            gf = gf - dJ_soft(grid, x_cut, fermion=True)
+        cs_b = CubicSpline(grid, gb, bc_type="not-a-knot")
+        cs_f = CubicSpline(grid, gf, bc_type="not-a-knot")
         _SPLINE_CACHE[key] = DJSpline(
             cb=np.ascontiguousarray(cs_b.c, dtype=np.float64),
             cf=np.ascontiguousarray(cs_f.c, dtype=np.float64),
@@ -198,35 +243,37 @@ def vprime(phi, T: float, p: ModelParams, sp: Optional[DJSpline] = None) -> np.n
     return dV
 
 
-def vprime_scalar(phi: float, T: float, p: ModelParams) -> float:
-    return float(vprime(np.array([phi]), T, p)[0])
+def vprime_scalar(phi: float, T: float, p: ModelParams, sp: Optional[DJSpline] = None) -> float:
+    return float(vprime(np.array([phi]), T, p, sp)[0])
 
 
 def build_vprime_table(T: float, phi_min: float, phi_max: float, n_table: int,
-                       p: ModelParams) -> Tuple[np.ndarray, float, float]:
+                       p: ModelParams, sp: Optional[DJSpline] = None
+                       ) -> Tuple[np.ndarray, float, float]:
     """(table, tmin, dinv) with table[i] = V'(phi_min + i*dphi) (latticeSimeRescale_numba.py:840-849)."""
     grid = np.linspace(phi_min, phi_max, n_table, dtype=np.float64)
-    table = vprime(grid, T, p)
+    table = vprime(grid, T, p, sp)
     return table, float(phi_min), 1.0 / ((phi_max - phi_min) / (n_table - 1))
 
 
-def thermal_vev(T: float, p: ModelParams) -> float:
+def thermal_vev(T: float, p: ModelParams, sp: Optional[DJSpline] = None) -> float:
     """brentq root of V' on [1e-3 vev, 2 vev] (latticeSimeRescale_numba.py:852-863)."""
     v = p.vev_tree
     try:
-        return brentq(lambda x: vprime_scalar(x, T, p), v * 0.001, v * 2.0)
+        return brentq(lambda x: vprime_scalar(x, T, p, sp), v * 0.001, v * 2.0)
     except ValueError:
         return v
 
 
-def barrier_top(T: float, p: ModelParams, vev_th: Optional[float] = None) -> Optional[float]:
+def barrier_top(T: float, p: ModelParams, vev_th: Optional[float] = None,
+                sp: Optional[DJSpline] = None) -> Optional[float]:
     """First + -> - sign change of V' on the log grid of latticeSimeRescale_numba.py:877-917."""
-    vev_th = vev_th or thermal_vev(T, p)
+    vev_th = vev_th or thermal_vev(T, p, sp)
     pts = np.concatenate([[1e-10], np.logspace(0, np.log10(vev_th * 0.99), 500)])
-    vals = vprime(pts, T, p)
+    vals = vprime(pts, T, p, sp)
     for i in range(len(pts) - 1):
         if vals[i] > 0 and vals[i + 1] < 0:
-            return brentq(lambda x: vprime_scalar(x, T, p), pts[i], pts[i + 1])
+            return brentq(lambda x: vprime_scalar(x, T, p, sp), pts[i], pts[i + 1])
     return None
 
 
