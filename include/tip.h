@@ -228,10 +228,16 @@ int tip_lattice_download(tip_lattice *lat, int which, void *host);
 int tip_lattice_set_table(tip_lattice *lat, const double *host_table, int64_t npts,
                           double tmin, double dinv, int odd);
 int tip_lattice_set_zn(tip_lattice *lat, double zn_n, double zn_dv, double zn_active);
-/* Launch geometry of the float32 step kernels (the autotuner's knobs): rows of 64 z-sites per
- * block (1, 2, 4, 8; 0 = default), x planes each block marches through (0 = derived), and the
- * XCD-aware block remap (0/1). */
-int tip_lattice_set_geometry(tip_lattice *lat, int rows_per_block, int planes_per_block,
+/* Launch geometry of the float32 step kernels (the autotuner's knobs).  kernel:
+ * TIP_KERNEL_AUTO (row tiles where they apply, else flat tiles), TIP_KERNEL_ROWS (real field:
+ * whole z rows staged in LDS, `rows` = y rows per thread, 1, 2 or 4, dividing ny; needs
+ * nz <= 1024) or TIP_KERNEL_FLAT (`rows` = rows of 64 z-sites per block, 1, 2, 4 or 8; the
+ * complex field always uses flat tiles).  rows = 0 picks the default, planes_per_block = x
+ * planes each block marches through (0 = derived), xcd_remap = XCD-aware block order (0/1). */
+#define TIP_KERNEL_AUTO 0
+#define TIP_KERNEL_ROWS 1
+#define TIP_KERNEL_FLAT 2
+int tip_lattice_set_geometry(tip_lattice *lat, int kernel, int rows, int planes_per_block,
                              int xcd_remap);
 /* nsteps RK2 steps; elapsed_ms (nullable) receives the HIP-event time of the launches on the
  * lattice's stream and makes the call synchronous. */

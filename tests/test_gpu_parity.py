@@ -300,19 +300,23 @@ def test_lattice_geometries_bitwise_identical():
     kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
               noise_scale=prm["noise_scale"])
     ref = None
-    for by, cx, xcd in [(0, 0, 1), (1, 3, 0), (2, 7, 1), (4, 40, 0), (8, 1, 1), (8, 16, 0)]:
+    # (rows, planes, xcd, kernel): auto, flat tiles, LDS row tiles (nz = 96: half-idle wave)
+    for by, cx, xcd, kind in [(0, 0, 1, 0), (1, 3, 0, 2), (2, 7, 1, 2), (4, 40, 0, 2),
+                              (8, 1, 1, 2), (8, 16, 0, 2), (1, 5, 1, 1), (2, 3, 0, 1),
+                              (4, 40, 1, 1), (2, 1, 1, 1)]:
         lat = Lattice(shape, np.float32)
         lat.upload("phi", phi0)
         lat.upload("pi", pi0)
         lat.set_table(t, tmin, dinv, odd=True)
-        lat.set_geometry(by, cx, xcd)
+        lat.set_geometry(by, cx, xcd, kind)
         lat.step(5, seed=3, **kw)
         out = (lat.download("phi"), lat.download("pi"))
         lat.close()
         if ref is None:
             ref = out
         else:
-            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (by, cx, xcd)
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), \
+                (by, cx, xcd, kind)
     # and the drop-in entry point gives the same numbers
     from tip import kernels as K
     g, h = phi0.copy(), pi0.copy()

@@ -15,6 +15,8 @@
 // Each plane costs two new HBM loads per site (phi, pi at x+1); the four y/z neighbours are
 // row segments that the same or adjacent waves just loaded, served from L1/L2.
 #pragma once
+#include <type_traits>
+
 #include "tip_internal.h"
 
 namespace tip {
@@ -22,8 +24,14 @@ namespace tip {
 template <typename R>
 struct StepScalars {
     R dt, half_dt, inv_dx2, inv_mu2, eta, inv_a2, ns;
+    R c_lap;  // inv_a2 * inv_dx2 (float32 path)
+    R ns_l2;  // ns * sqrt(2 ln 2): amplitude applied to sqrt(-log2 u1) cos(2 pi u2)
     uint32_t word_mask;  // all ones (see unit_f32)
 };
+
+// Noise source of the fused step, a compile-time choice so that no branch joins the three
+// paths inside the plane loop.
+enum NoiseMode { NOISE_NONE = 0, NOISE_ARRAY = 1, NOISE_HASH = 2 };
 
 template <typename R>
 inline StepScalars<R> make_scalars(double dt, double dx, double eta_eff, double mu,
@@ -37,6 +45,8 @@ inline StepScalars<R> make_scalars(double dt, double dx, double eta_eff, double 
     c.eta = (R)eta_eff;
     c.inv_a2 = (R)inv_a2;
     c.ns = (R)noise_scale;
+    c.c_lap = (R)(inv_a2 / (dx * dx));
+    c.ns_l2 = (R)(noise_scale * 1.1774100225154747);  // sqrt(2 ln 2)
     c.word_mask = 0xFFFFFFFFu;
     return c;
 }
@@ -164,6 +174,333 @@ __device__ __forceinline__ unsigned rk2_real_tile(
     return cnt;
 }
 
+// ---------------------------------------------------------------------------------------
+// float32 tile: the same step, rearranged for the float32 path's instruction count and
+// software-pipelined over the x march.
+//
+//  * lap(phi_t) = lap(phi) + dt/2 lap(pi): the stage-2 numerator is t1 + dt/2 (sum6(pi) - 6 pi)
+//    (algebraically the reference's sum of the six neighbours' phi_t minus 6 phi_t);
+//    inv_a2 * inv_dx2 is one constant.
+//  * Both table gathers depend only on (phi, pi) of the current plane, so they are issued
+//    first; the loads of the next plane (two centre values, eight neighbours) follow, and the
+//    hash noise is computed while all of them are in flight.  Loads complete in order, so the
+//    gathers are waited for without waiting for the next plane.
+//  * noise: ns * sqrt(-2 ln u1) cos(2 pi u2) = ns sqrt(2 ln 2) * sqrt(-log2 u1) * cos(u2 rev),
+//    u2 from the high word of the second state (its low word moves u2 by < 2^-32).
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float gauss_f32_l2(uint64_t seed, uint32_t word_mask)
+{
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    const float u1 = fmaxf(unit_f32(x1, word_mask), 1e-12f);
+    const float u2 = (float)((uint32_t)(x2 >> 32) & word_mask) * 0x1p-32f;
+    return __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
+}
+
+template <bool ODD>
+__device__ __forceinline__ float2 seg_load(__amdgpu_buffer_rsrc_t r, const TableF32<ODD>& t,
+                                           float x)
+{
+    const float ax = ODD ? fabsf(x) : x;
+    const float fidx = fmaf(ax, t.dinv, t.off);
+    const uint32_t idx = (uint32_t)(int)__builtin_amdgcn_fmed3f(fidx, 0.0f, (float)t.last);
+    using f2 = float __attribute__((ext_vector_type(2)));
+    const f2 v = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), 0, 0));
+    return make_float2(v.x, v.y);
+}
+
+template <bool ODD>
+__device__ __forceinline__ float seg_eval(float2 s, float x)
+{
+    if (!ODD) return fmaf(s.y, x, s.x);
+    const float w = fmaf(s.y, fabsf(x), s.x);
+    return __uint_as_float(__float_as_uint(w) ^ (__float_as_uint(x) & 0x80000000u));
+}
+
+template <bool ODD, int BY, int NM>
+__device__ __forceinline__ unsigned rk2_real_tile_f32(
+    const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
+    float* __restrict__ pi_out, const float* __restrict__ noise, const StepScalars<float>& c,
+    const TableF32<ODD>& tab, int nx, int ny, int nz, int tile_p, int x0, int x1,
+    uint64_t noise_seed, bool count, float esc)
+{
+    const int nyz = ny * nz;
+    const int pl = tile_p * (64 * BY) + (int)threadIdx.y * 64 + (int)threadIdx.x;
+    const bool act = pl < nyz;
+    const int p0 = act ? pl : 0;
+    const int jj = p0 / nz, kk = p0 - jj * nz;
+    const int kp = kk + 1 < nz ? kk + 1 : 0, km = kk - 1 >= 0 ? kk - 1 : nz - 1;
+    const int jp = jj + 1 < ny ? jj + 1 : 0, jm = jj - 1 >= 0 ? jj - 1 : ny - 1;
+    const uint32_t o_c = (uint32_t)(jj * nz + kk) * 4u;
+    const uint32_t o_kp = (uint32_t)(jj * nz + kp) * 4u, o_km = (uint32_t)(jj * nz + km) * 4u;
+    const uint32_t o_jp = (uint32_t)(jp * nz + kk) * 4u, o_jm = (uint32_t)(jm * nz + kk) * 4u;
+    const size_t plane = (size_t)nyz * 4u;
+    const uint32_t pb = (uint32_t)plane;
+    const char* P = reinterpret_cast<const char*>(phi);
+    const char* Q = reinterpret_cast<const char*>(pi);
+    const char* PO = reinterpret_cast<const char*>(phi_out);
+    const char* QO = reinterpret_cast<const char*>(pi_out);
+    const char* NZ = reinterpret_cast<const char*>(noise);
+    auto at = [&](const char* b, int x) { return make_rsrc(b + (size_t)x * plane, pb); };
+    const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
+    const uint64_t colK = (uint64_t)(uint32_t)(jj * nz + kk) * (uint64_t)kNoiseMult;
+    const uint64_t planeK = (uint64_t)(uint32_t)nyz * (uint64_t)kNoiseMult;
+    uint64_t xK = (uint64_t)(uint32_t)x0 * planeK;
+
+    unsigned cnt = 0;
+    const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
+    const int xq = x0 + 1 < nx ? x0 + 1 : 0;
+    // Window of centre values at x-1, x, x+1 and the prefetch slot for x+2 (w*), and two sets
+    // of the eight in-plane neighbours (current plane / next plane).
+    float wp0 = buf_load<float>(at(P, xm), o_c), wq0 = buf_load<float>(at(Q, xm), o_c);
+    float wp1 = buf_load<float>(at(P, x0), o_c), wq1 = buf_load<float>(at(Q, x0), o_c);
+    float wp2 = buf_load<float>(at(P, xq), o_c), wq2 = buf_load<float>(at(Q, xq), o_c);
+    float wp3, wq3;
+    float A[8], B[8];  // [phi y+, y-, z+, z-, pi y+, y-, z+, z-]
+    {
+        const __amdgpu_buffer_rsrc_t Pb = at(P, x0), Qb = at(Q, x0);
+        A[0] = buf_load<float>(Pb, o_jp); A[1] = buf_load<float>(Pb, o_jm);
+        A[2] = buf_load<float>(Pb, o_kp); A[3] = buf_load<float>(Pb, o_km);
+        A[4] = buf_load<float>(Qb, o_jp); A[5] = buf_load<float>(Qb, o_jm);
+        A[6] = buf_load<float>(Qb, o_kp); A[7] = buf_load<float>(Qb, o_km);
+    }
+
+    // One plane.  The next plane's loads are issued unconditionally (periodic indices keep
+    // them in bounds); on the chunk's last plane they are simply not used.
+    auto plane_step = [&](int x, float pm, float qm, float pc, float qc, float pp, float qp,
+                          float& pn2, float& qn2, const float (&cur)[8], float (&nxt)[8]) {
+        const float ftc = fmaf(c.half_dt, qc, pc);
+        const float2 s1 = seg_load<ODD>(T, tab, pc);
+        const float2 s2 = seg_load<ODD>(T, tab, ftc);
+        float nzv = 0.0f;
+        if (NM == NOISE_ARRAY) nzv = buf_load<float>(at(NZ, x), o_c);
+        {
+            const int xa = x + 1 < nx ? x + 1 : 0;
+            const int xb = xa + 1 < nx ? xa + 1 : 0;
+            pn2 = buf_load<float>(at(P, xb), o_c);
+            qn2 = buf_load<float>(at(Q, xb), o_c);
+            const __amdgpu_buffer_rsrc_t Pa = at(P, xa), Qa = at(Q, xa);
+            nxt[0] = buf_load<float>(Pa, o_jp); nxt[1] = buf_load<float>(Pa, o_jm);
+            nxt[2] = buf_load<float>(Pa, o_kp); nxt[3] = buf_load<float>(Pa, o_km);
+            nxt[4] = buf_load<float>(Qa, o_jp); nxt[5] = buf_load<float>(Qa, o_jm);
+            nxt[6] = buf_load<float>(Qa, o_kp); nxt[7] = buf_load<float>(Qa, o_km);
+        }
+        float g = 0.0f;
+        if (NM == NOISE_HASH) g = gauss_f32_l2((colK + xK) ^ noise_seed, c.word_mask);
+
+        const float s6 = ((((pp + pm) + cur[0]) + cur[1]) + cur[2]) + cur[3];
+        const float t1 = fmaf(-6.0f, pc, s6);
+        const float q6 = ((((qp + qm) + cur[4]) + cur[5]) + cur[6]) + cur[7];
+        const float tq = fmaf(-6.0f, qc, q6);
+        const float t2 = fmaf(c.half_dt, tq, t1);
+        const float k1 = fmaf(-c.inv_mu2, seg_eval<ODD>(s1, pc), fmaf(c.c_lap, t1, -c.eta * qc));
+        const float ptc = fmaf(c.half_dt, k1, qc);
+        const float k2 =
+            fmaf(-c.inv_mu2, seg_eval<ODD>(s2, ftc), fmaf(c.c_lap, t2, -c.eta * ptc));
+        const float pn = fmaf(c.dt, ptc, pc);
+        float qn;
+        if (NM == NOISE_HASH) qn = fmaf(c.dt, k2, fmaf(c.ns_l2, g, qc));
+        else if (NM == NOISE_ARRAY) qn = qc + fmaf(c.dt, k2, nzv);
+        else qn = fmaf(c.dt, k2, qc);
+        if (act) {
+            buf_store(pn, at(PO, x), o_c);
+            buf_store(qn, at(QO, x), o_c);
+        }
+        if (count) cnt += wave_count(act && fabsf(pn) > esc);
+        xK += planeK;
+    };
+
+    // Four planes per trip: the window and neighbour sets change roles instead of moving.
+    int x = x0;
+    for (; x + 4 <= x1; x += 4) {
+        plane_step(x + 0, wp0, wq0, wp1, wq1, wp2, wq2, wp3, wq3, A, B);
+        plane_step(x + 1, wp1, wq1, wp2, wq2, wp3, wq3, wp0, wq0, B, A);
+        plane_step(x + 2, wp2, wq2, wp3, wq3, wp0, wq0, wp1, wq1, A, B);
+        plane_step(x + 3, wp3, wq3, wp0, wq0, wp1, wq1, wp2, wq2, B, A);
+    }
+    for (; x < x1; ++x) {
+        plane_step(x, wp0, wq0, wp1, wq1, wp2, wq2, wp3, wq3, A, B);
+        wp0 = wp1; wq0 = wq1; wp1 = wp2; wq1 = wq2; wp2 = wp3; wq2 = wq3;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) A[k] = B[k];
+    }
+    return cnt;
+}
+
+// ---------------------------------------------------------------------------------------
+// float32 row-tile kernel with LDS staging (the default float32 real-field step).
+//
+// A group of W = ceil(nz/64) waves covers whole z rows (lane z of wave zw is site
+// z = 64 zw + lane), and each thread carries TY consecutive y rows of its z column through
+// the x march.  Per plane, the group stages its TY rows of (phi, pi) in LDS once, behind one
+// barrier (double-buffered, so the next plane's staging needs no second barrier); z
+// neighbours, periodic inside the row, are LDS reads; y neighbours are registers of the same
+// thread except for the two halo rows, which are loaded; x neighbours are the register window.
+// That is 2 + 4/TY global loads per site against 10 for the flat tile above, which is what
+// bounds the flat tile (its y/z neighbours are L1/L2 requests).  A block holds G groups of
+// consecutive row blocks (G = 4/W for narrow rows, so a block has 4 waves).  TY must divide
+// ny; groups past the last row are idle.  The next plane's centres and halo rows are
+// loaded one plane ahead.
+// ---------------------------------------------------------------------------------------
+template <bool ODD, int TY, int NM>
+__device__ __forceinline__ unsigned rk2_real_rows_tile(
+    const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
+    float* __restrict__ pi_out, const float* __restrict__ noise, const StepScalars<float>& c,
+    const TableF32<ODD>& tab, int nx, int ny, int nz, int W, int tile, int x0, int x1,
+    uint64_t noise_seed, bool count, float esc, float* __restrict__ lds)
+{
+    // LDS words per z column: 2 buffers x 2 fields x TY rows, plus one pad word so that
+    // consecutive lanes (stride K words) hit distinct banks.  Every slot of a column is a
+    // compile-time offset from one of three per-thread base addresses (z, z+1, z-1).
+    constexpr int K = 4 * TY + 1;
+    const int G = (int)blockDim.y / W;
+    const int w = (int)threadIdx.y, grp = w / W, zw = w - grp * W;
+    const int z = zw * 64 + (int)threadIdx.x;
+    const bool zact = z < nz;
+    const int zc = zact ? z : 0;
+    const int zp = zc + 1 < nz ? zc + 1 : 0, zm = zc > 0 ? zc - 1 : nz - 1;
+    const int y0 = (tile * G + grp) * TY;
+    const bool act = zact && y0 < ny;
+    const int yb = y0 < ny ? y0 : 0;
+    float* Lg = lds + (size_t)grp * (64 * W) * K;
+    float* Lz = Lg + z * K;
+    const float* Lzp = Lg + zp * K;
+    const float* Lzm = Lg + zm * K;
+
+    const int nyz = ny * nz;
+    const size_t plane = (size_t)nyz * 4u;
+    const uint32_t pb = (uint32_t)plane;
+    const char* P = reinterpret_cast<const char*>(phi);
+    const char* Q = reinterpret_cast<const char*>(pi);
+    const char* PO = reinterpret_cast<const char*>(phi_out);
+    const char* QO = reinterpret_cast<const char*>(pi_out);
+    const char* NZ = reinterpret_cast<const char*>(noise);
+    auto at = [&](const char* b, int x) { return make_rsrc(b + (size_t)x * plane, pb); };
+    const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
+    uint32_t off[TY], offs[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        off[r] = (uint32_t)((yb + r) * nz + zc) * 4u;
+        // idle lanes store past the buffer end, where buffer stores are dropped: no branch
+        offs[r] = act ? off[r] : 0xFFFFFFF0u;
+    }
+    const uint32_t offm = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz + zc) * 4u;
+    const uint32_t offp = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz + zc) * 4u;
+    const uint64_t planeK = (uint64_t)(uint32_t)nyz * (uint64_t)kNoiseMult;
+    const uint64_t col0K = (uint64_t)(uint32_t)(yb * nz + zc) * (uint64_t)kNoiseMult;
+    const uint64_t rowK = (uint64_t)(uint32_t)nz * (uint64_t)kNoiseMult;
+    uint64_t xK = (uint64_t)(uint32_t)x0 * planeK;
+
+    // Centre window x-1, x, x+1 and the slot for x+2 (w0..w3 change roles each plane), and
+    // halo rows (y0-1, y0+TY) of the current / next plane.
+    float p0[TY], q0[TY], p1[TY], q1[TY], p2[TY], q2[TY], p3[TY], q3[TY];
+    {
+        const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
+        const int xq = x0 + 1 < nx ? x0 + 1 : 0;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            p0[r] = buf_load<float>(at(P, xm), off[r]); q0[r] = buf_load<float>(at(Q, xm), off[r]);
+            p1[r] = buf_load<float>(at(P, x0), off[r]); q1[r] = buf_load<float>(at(Q, x0), off[r]);
+            p2[r] = buf_load<float>(at(P, xq), off[r]); q2[r] = buf_load<float>(at(Q, xq), off[r]);
+        }
+    }
+    float hA[4], hB[4];  // phi y-1, pi y-1, phi y+TY, pi y+TY
+    hA[0] = buf_load<float>(at(P, x0), offm); hA[1] = buf_load<float>(at(Q, x0), offm);
+    hA[2] = buf_load<float>(at(P, x0), offp); hA[3] = buf_load<float>(at(Q, x0), offp);
+    unsigned cnt = 0;
+
+    auto plane_step = [&](auto bufc, int x, const float (&pm)[TY], const float (&qm)[TY],
+                          const float (&pc)[TY], const float (&qc)[TY], const float (&pp)[TY],
+                          const float (&qp)[TY], float (&pn)[TY], float (&qn)[TY],
+                          const float (&hc)[4], float (&hn)[4]) {
+        constexpr int b = decltype(bufc)::value;
+        constexpr int oP = (b * 2 + 0) * TY, oQ = (b * 2 + 1) * TY;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            Lz[oP + r] = pc[r];
+            Lz[oQ + r] = qc[r];
+        }
+        {
+            const int xa = x + 1 < nx ? x + 1 : 0;
+            const int xb = xa + 1 < nx ? xa + 1 : 0;
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                pn[r] = buf_load<float>(at(P, xb), off[r]);
+                qn[r] = buf_load<float>(at(Q, xb), off[r]);
+            }
+            hn[0] = buf_load<float>(at(P, xa), offm); hn[1] = buf_load<float>(at(Q, xa), offm);
+            hn[2] = buf_load<float>(at(P, xa), offp); hn[3] = buf_load<float>(at(Q, xa), offp);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const float pyp = r + 1 < TY ? pc[r + 1 < TY ? r + 1 : r] : hc[2];
+            const float pym = r > 0 ? pc[r > 0 ? r - 1 : 0] : hc[0];
+            const float qyp = r + 1 < TY ? qc[r + 1 < TY ? r + 1 : r] : hc[3];
+            const float qym = r > 0 ? qc[r > 0 ? r - 1 : 0] : hc[1];
+            const float pzp = Lzp[oP + r], pzm = Lzm[oP + r];
+            const float qzp = Lzp[oQ + r], qzm = Lzm[oQ + r];
+            const float ftc = fmaf(c.half_dt, qc[r], pc[r]);
+            const float2 s1 = seg_load<ODD>(T, tab, pc[r]);
+            const float2 s2 = seg_load<ODD>(T, tab, ftc);
+            float nzv = 0.0f, g = 0.0f;
+            if (NM == NOISE_ARRAY) nzv = buf_load<float>(at(NZ, x), off[r]);
+            if (NM == NOISE_HASH)
+                g = gauss_f32_l2((col0K + (uint64_t)r * rowK + xK) ^ noise_seed, c.word_mask);
+            const float s6 = ((((pp[r] + pm[r]) + pyp) + pym) + pzp) + pzm;
+            const float t1 = fmaf(-6.0f, pc[r], s6);
+            const float q6 = ((((qp[r] + qm[r]) + qyp) + qym) + qzp) + qzm;
+            const float tq = fmaf(-6.0f, qc[r], q6);
+            const float t2 = fmaf(c.half_dt, tq, t1);
+            const float k1 =
+                fmaf(-c.inv_mu2, seg_eval<ODD>(s1, pc[r]), fmaf(c.c_lap, t1, -c.eta * qc[r]));
+            const float ptc = fmaf(c.half_dt, k1, qc[r]);
+            const float k2 =
+                fmaf(-c.inv_mu2, seg_eval<ODD>(s2, ftc), fmaf(c.c_lap, t2, -c.eta * ptc));
+            const float pnew = fmaf(c.dt, ptc, pc[r]);
+            float qnew;
+            if (NM == NOISE_HASH) qnew = fmaf(c.dt, k2, fmaf(c.ns_l2, g, qc[r]));
+            else if (NM == NOISE_ARRAY) qnew = qc[r] + fmaf(c.dt, k2, nzv);
+            else qnew = fmaf(c.dt, k2, qc[r]);
+            buf_store(pnew, at(PO, x), offs[r]);
+            buf_store(qnew, at(QO, x), offs[r]);
+            if (count) cnt += wave_count(act && fabsf(pnew) > esc);
+        }
+        xK += planeK;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    int x = x0;
+    for (; x + 4 <= x1; x += 4) {
+        plane_step(B0{}, x + 0, p0, q0, p1, q1, p2, q2, p3, q3, hA, hB);
+        plane_step(B1{}, x + 1, p1, q1, p2, q2, p3, q3, p0, q0, hB, hA);
+        plane_step(B0{}, x + 2, p2, q2, p3, q3, p0, q0, p1, q1, hA, hB);
+        plane_step(B1{}, x + 3, p3, q3, p0, q0, p1, q1, p2, q2, hB, hA);
+    }
+    // remainder: canonical roles, explicit rotation; buffers keep alternating
+    auto rotate = [&]() {
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            p0[r] = p1[r]; q0[r] = q1[r]; p1[r] = p2[r]; q1[r] = q2[r];
+            p2[r] = p3[r]; q2[r] = q3[r];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hA[k] = hB[k];
+    };
+    if (x < x1) {
+        plane_step(B0{}, x, p0, q0, p1, q1, p2, q2, p3, q3, hA, hB);
+        rotate();
+        ++x;
+    }
+    if (x < x1) {
+        plane_step(B1{}, x, p0, q0, p1, q1, p2, q2, p3, q3, hA, hB);
+        rotate();
+        ++x;
+    }
+    if (x < x1) plane_step(B0{}, x, p0, q0, p1, q1, p2, q2, p3, q3, hA, hB);
+    return cnt;
+}
+
 // Block -> tile decode.  Blocks are numbered with the z tile fastest, then the y tile, then
 // the x chunk.  With `xcd_remap` the hardware's round-robin dispatch of consecutive block ids
 // over the 8 XCDs is undone: block b runs tile (b % 8) * (nb / 8) + b / 8, so each XCD works
@@ -193,6 +530,23 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_real(
     unsigned cnt = rk2_real_tile<R, Tab, BY>(phi, pi, phi_out, pi_out, noise, c, tab, nx, ny, nz,
                                              tp, x0, x1, noise_seed, hash_noise, vac != nullptr,
                                              esc);
+    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+}
+
+template <bool ODD, int BY, int NM>
+__global__ __launch_bounds__(64 * BY) void k_rk2_real_f32(
+    const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
+    float* __restrict__ pi_out, const float* __restrict__ noise, StepScalars<float> c,
+    TableF32<ODD> tab, int nx, int ny, int nz, TileGrid g, uint64_t noise_seed,
+    unsigned long long* __restrict__ vac, float esc)
+{
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
+    unsigned cnt = rk2_real_tile_f32<ODD, BY, NM>(phi, pi, phi_out, pi_out, noise, c, tab, nx,
+                                                  ny, nz, tp, x0, x1, noise_seed, vac != nullptr,
+                                                  esc);
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
@@ -367,12 +721,42 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
+template <bool ODD, int TY, int NM>
+__global__ __launch_bounds__(1024) void k_rk2_real_rows(
+    const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
+    float* __restrict__ pi_out, const float* __restrict__ noise, StepScalars<float> c,
+    TableF32<ODD> tab, int nx, int ny, int nz, int W, TileGrid g, uint64_t noise_seed,
+    unsigned long long* __restrict__ vac, float esc)
+{
+    extern __shared__ float lds_rows[];
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
+    unsigned cnt = rk2_real_rows_tile<ODD, TY, NM>(phi, pi, phi_out, pi_out, noise, c, tab, nx,
+                                                   ny, nz, W, tp, x0, x1, noise_seed,
+                                                   vac != nullptr, esc, lds_rows);
+    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+}
+
 // Launch geometry shared by the real and complex launchers.
+//   kind TIP_KERNEL_ROWS (real float32 default where it applies): by = y rows per thread
+//   (1, 2 or 4; must divide ny), cx = x planes per block.
+//   kind TIP_KERNEL_FLAT: by = rows of 64 z-sites per block (1, 2, 4 or 8), cx as above.
 struct StepGeometry {
-    int by;   // rows per block (1, 2, 4 or 8)
+    int by;   // see above (0 = default)
     int cx;   // x planes per block (0 = choose from the lattice size)
     int xcd;  // 1 = XCD-aware block remap
+    int kind; // TIP_KERNEL_AUTO / TIP_KERNEL_ROWS / TIP_KERNEL_FLAT
 };
+// Row-tile shape of a real float32 lattice: waves per row W, groups per block G, and whether
+// the row kernel applies at all (nz <= 1024, ny divisible by TY).
+struct RowShape {
+    int W, G, TY;
+    bool ok;
+};
+RowShape row_shape(int64_t ny, int64_t nz, int ty);
+int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz);
 StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field, int f64);
 // Resolve a requested geometry (nullable = default) into the tile grid and the block count.
 int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz,

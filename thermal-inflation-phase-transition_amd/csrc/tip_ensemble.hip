@@ -22,7 +22,7 @@ struct EnsInst {
     float* f[2][2];  // [parity][phi, pi]
     const float2* ab;
     StepScalars<float> c;
-    float tmin, dinv;
+    float tmin, dinv, off;
     int last, odd;
     int nx, ny, nz;
     uint64_t seed_mix;
@@ -57,11 +57,13 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_ensemble(
         tab.tmin = I.tmin;
         tab.dinv = I.dinv;
         tab.last = I.last;
+        tab.off = I.off;
         const int x1 = min(it.x0 + cx, I.nx);
-        unsigned cnt = rk2_real_tile<float, TableF32<ODD>, BY>(
+        // Instances without noise have ns = 0: fma(0, g, pi) = pi, so the hash path gives the
+        // noise-free result bit for bit.
+        unsigned cnt = rk2_real_tile_f32<ODD, BY, NOISE_HASH>(
             I.f[parity][0], I.f[parity][1], I.f[parity ^ 1][0], I.f[parity ^ 1][1], nullptr, I.c,
-            tab, I.nx, I.ny, I.nz, it.tp, it.x0, x1, step ^ I.seed_mix, I.hash,
-            vac != nullptr, esc);
+            tab, I.nx, I.ny, I.nz, it.tp, it.x0, x1, step ^ I.seed_mix, vac != nullptr, esc);
         if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
     }
 }
@@ -326,6 +328,7 @@ extern "C" int tip_ensemble_set_table(tip_ensemble* E, int i, const double* host
     I.ab = E->table_ab[i];
     I.tmin = (float)tmin;
     I.dinv = (float)dinv;
+    I.off = (float)(-tmin * dinv);
     I.last = (int)(npts - 2);
     I.odd = odd ? 1 : 0;
     E->has_table[i] = 1;
@@ -385,6 +388,7 @@ extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, floa
                 tab.dinv = I.dinv;
                 tab.last = I.last;
                 tab.odd = I.odd;
+                tab.off = I.off;
                 rc = launch_real_f32(I.f[par][0], I.f[par][1], I.f[par ^ 1][0], I.f[par ^ 1][1],
                                      I.c, nullptr, tab, I.nx, I.ny, I.nz,
                                      (E->step + (uint64_t)s) ^ I.seed_mix, I.hash, nullptr, 0.0f,

@@ -216,6 +216,7 @@ struct TableF32 {
     const float2* ab;
     float tmin, dinv;
     int last;
+    float off;  // -tmin * dinv: fidx = x * dinv + off in one fma
     __device__ __forceinline__ float operator()(float x) const
     {
         const float ax = ODD ? fabsf(x) : x;
@@ -235,6 +236,7 @@ struct SegTable {
     float tmin, dinv;
     int last;
     int odd;
+    float off;
     template <bool ODD>
     TableF32<ODD> as() const
     {
@@ -243,6 +245,7 @@ struct SegTable {
         t.tmin = tmin;
         t.dinv = dinv;
         t.last = last;
+        t.off = off;
         return t;
     }
 };

@@ -20,7 +20,7 @@ struct tip_lattice {
     double zn_n = 0.0, zn_dv = 0.0, zn_active = 0.0;
     unsigned long long* counter = nullptr;  // 2 entries
     double* mm = nullptr;                   // 2 entries
-    tip::StepGeometry geom = {0, 0, 1};     // by = 0: the default for the lattice size
+    tip::StepGeometry geom = {0, 0, 1, TIP_KERNEL_AUTO};  // defaults for the lattice size
 };
 
 using namespace tip;
@@ -176,17 +176,36 @@ extern "C" int tip_lattice_set_zn(tip_lattice* lat, double zn_n, double zn_dv, d
     return TIP_OK;
 }
 
-extern "C" int tip_lattice_set_geometry(tip_lattice* lat, int rows_per_block,
+extern "C" int tip_lattice_set_geometry(tip_lattice* lat, int kernel, int rows,
                                         int planes_per_block, int xcd_remap)
 {
     TIP_REQUIRE(lat, "tip_lattice_set_geometry: NULL");
-    StepGeometry req = {rows_per_block, planes_per_block, xcd_remap};
-    StepGeometry g;
-    TileGrid tg;
-    int nb;
-    int rc = resolve_geometry(&req, lat->nx, lat->ny, lat->nz, lat->kind == TIP_COMPLEX, 0, g, tg,
-                              nb);
-    if (rc) return rc;
+    TIP_REQUIRE(kernel == TIP_KERNEL_AUTO || kernel == TIP_KERNEL_ROWS ||
+                    kernel == TIP_KERNEL_FLAT,
+                "tip_lattice_set_geometry: unknown kernel %d", kernel);
+    StepGeometry req = {rows, planes_per_block, xcd_remap, kernel};
+    if (kernel == TIP_KERNEL_ROWS) {
+        TIP_REQUIRE(lat->kind == TIP_REAL && lat->prec == 4,
+                    "tip_lattice_set_geometry: the row kernel is the real float32 step");
+        const RowShape rs = row_shape(
+            lat->ny, lat->nz, rows ? rows : default_rows_per_thread(lat->nx, lat->ny, lat->nz));
+        TIP_REQUIRE(rs.ok, "tip_lattice_set_geometry: row kernel needs nz <= 1024 and rows per "
+                           "thread (1, 2, 4) dividing ny");
+        TIP_REQUIRE(planes_per_block >= 0 && planes_per_block <= 4096,
+                    "geometry: planes per block out of range");
+    } else {
+        StepGeometry g;
+        TileGrid tg;
+        int nb;
+        StepGeometry flat = req;
+        // under AUTO a rows value the flat kernel does not take is a row-kernel value
+        if (kernel == TIP_KERNEL_AUTO && rows != 0 && rows != 1 && rows != 2 && rows != 4 &&
+            rows != 8)
+            TIP_REQUIRE(false, "geometry: rows must be 0, 1, 2, 4 or 8 (got %d)", rows);
+        int rc = resolve_geometry(&flat, lat->nx, lat->ny, lat->nz, lat->kind == TIP_COMPLEX, 0,
+                                  g, tg, nb);
+        if (rc) return rc;
+    }
     lat->geom = req;
     return TIP_OK;
 }

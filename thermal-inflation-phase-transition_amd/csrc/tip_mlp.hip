@@ -354,7 +354,7 @@ int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, c
     StepGeometry g;
     TileGrid tg;
     int nb;
-    StepGeometry req = {1, 0, 1};
+    StepGeometry req = {1, 0, 1, TIP_KERNEL_FLAT};
     rc = resolve_geometry(&req, nx, ny, nz, 1, 0, g, tg, nb);
     if (rc) return rc;
     hipLaunchKernelGGL((k_rk2_complex<float, MlpForce, 1>), dim3(nb), dim3(64, 1), 0, st, phi1,

@@ -76,6 +76,28 @@ int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz
     return TIP_OK;
 }
 
+// Rows per thread of the row kernel: 4 on large lattices (fewer halo loads per site, measured
+// fastest at 256^3), 1 on small ones (more blocks to fill the machine); 4 and 2 need ny to be
+// divisible by them.
+int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz)
+{
+    if (nx * ny * nz >= (int64_t(1) << 23)) {
+        if (ny % 4 == 0) return 4;
+        if (ny % 2 == 0) return 2;
+    }
+    return 1;
+}
+
+RowShape row_shape(int64_t ny, int64_t nz, int ty)
+{
+    RowShape r;
+    r.W = (int)((nz + 63) / 64);
+    r.G = r.W >= 4 ? 1 : 4 / r.W;
+    r.TY = ty;
+    r.ok = r.W <= 16 && (ty == 1 || ty == 2 || ty == 4) && ny % ty == 0;
+    return r;
+}
+
 SegTable make_table32(const void* ab, double tmin, double dinv, int64_t npts, int odd)
 {
     SegTable tab;
@@ -84,6 +106,7 @@ SegTable make_table32(const void* ab, double tmin, double dinv, int64_t npts, in
     tab.dinv = (float)dinv;
     tab.last = (int)(npts - 2);
     tab.odd = odd ? 1 : 0;
+    tab.off = (float)(-tmin * dinv);
     return tab;
 }
 
@@ -108,21 +131,81 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
     if (rc) return rc;
     TIP_REQUIRE(phi && pi && phi_out && pi_out && tab.ab, "rk2 f32: NULL field or table");
     TIP_REQUIRE(phi != phi_out && pi != pi_out, "rk2 f32: output must not alias input");
+    const int nm = noise ? NOISE_ARRAY : (hash ? NOISE_HASH : NOISE_NONE);
+    const int kind = geom ? geom->kind : TIP_KERNEL_AUTO;
+    TIP_REQUIRE(kind == TIP_KERNEL_AUTO || kind == TIP_KERNEL_ROWS || kind == TIP_KERNEL_FLAT,
+                "rk2 f32: unknown kernel kind %d", kind);
+    if (kind != TIP_KERNEL_FLAT) {
+        const int ty_req = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
+        const RowShape rs = row_shape(ny, nz, ty_req);
+        TIP_REQUIRE(rs.ok || kind == TIP_KERNEL_AUTO,
+                    "rk2 f32: row kernel needs nz <= 1024 and rows per thread (1, 2, 4) dividing "
+                    "ny (got %d rows, ny = %lld)", ty_req, (long long)ny);
+        if (rs.ok) {
+            TileGrid tg;
+            tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
+            // about 1024 blocks: 4 per CU, each marching enough planes to amortise its
+            // window fill and halo rows
+            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
+            if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
+            TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
+            tg.cx = (int)cx;
+            tg.chunks = (int)((nx + cx - 1) / cx);
+            const int64_t nb64 = (int64_t)tg.tiles * tg.chunks;
+            TIP_REQUIRE(nb64 < (int64_t(1) << 31), "geometry: too many blocks");
+            const int nbr = (int)nb64;
+            tg.xcd_remap = ((geom ? geom->xcd : 1) && nbr % 8 == 0) ? 1 : 0;
+            const size_t lds = (size_t)rs.G * (64 * rs.W) * (4 * rs.TY + 1) * sizeof(float);
+            const dim3 blk(64, rs.W * rs.G);
+#define TIP_LAUNCH_ROWS(TYV, ODD, NM)                                                           \
+    hipLaunchKernelGGL((k_rk2_real_rows<ODD, TYV, NM>), dim3(nbr), blk, lds, st, phi, pi,     \
+                       phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny, (int)nz,    \
+                       rs.W, tg, noise_seed, vac, esc)
+#define TIP_LAUNCH_ROWS_NM(TYV, ODD)                                      \
+    switch (nm) {                                                         \
+        case NOISE_ARRAY: TIP_LAUNCH_ROWS(TYV, ODD, NOISE_ARRAY); break;  \
+        case NOISE_HASH: TIP_LAUNCH_ROWS(TYV, ODD, NOISE_HASH); break;    \
+        default: TIP_LAUNCH_ROWS(TYV, ODD, NOISE_NONE); break;            \
+    }
+#define TIP_LAUNCH_ROWS_TY(ODD)                      \
+    switch (rs.TY) {                                 \
+        case 1: TIP_LAUNCH_ROWS_NM(1, ODD); break;   \
+        case 4: TIP_LAUNCH_ROWS_NM(4, ODD); break;   \
+        default: TIP_LAUNCH_ROWS_NM(2, ODD); break;  \
+    }
+            if (tab.odd) {
+                TIP_LAUNCH_ROWS_TY(true)
+            } else {
+                TIP_LAUNCH_ROWS_TY(false)
+            }
+#undef TIP_LAUNCH_ROWS_TY
+#undef TIP_LAUNCH_ROWS_NM
+#undef TIP_LAUNCH_ROWS
+            TIP_HIP(hipGetLastError());
+            return TIP_OK;
+        }
+    }
     StepGeometry g;
     TileGrid tg;
     int nb;
     rc = resolve_geometry(geom, nx, ny, nz, 0, 0, g, tg, nb);
     if (rc) return rc;
-#define TIP_LAUNCH_REAL(BYV, ODD)                                                              \
-    hipLaunchKernelGGL((k_rk2_real<float, TableF32<ODD>, BYV>), dim3(nb), dim3(64, BYV), 0, st, \
-                       phi, pi, phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny,     \
-                       (int)nz, tg, noise_seed, hash, vac, esc)
+#define TIP_LAUNCH_REAL(BYV, ODD, NM)                                                         \
+    hipLaunchKernelGGL((k_rk2_real_f32<ODD, BYV, NM>), dim3(nb), dim3(64, BYV), 0, st, phi,  \
+                       pi, phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny,        \
+                       (int)nz, tg, noise_seed, vac, esc)
+#define TIP_LAUNCH_REAL_NM(BYV, ODD)                                  \
+    switch (nm) {                                                     \
+        case NOISE_ARRAY: TIP_LAUNCH_REAL(BYV, ODD, NOISE_ARRAY); break; \
+        case NOISE_HASH: TIP_LAUNCH_REAL(BYV, ODD, NOISE_HASH); break;   \
+        default: TIP_LAUNCH_REAL(BYV, ODD, NOISE_NONE); break;           \
+    }
 #define TIP_LAUNCH_REAL_BY(ODD)                          \
     switch (g.by) {                                      \
-        case 1: TIP_LAUNCH_REAL(1, ODD); break;          \
-        case 2: TIP_LAUNCH_REAL(2, ODD); break;          \
-        case 8: TIP_LAUNCH_REAL(8, ODD); break;          \
-        default: TIP_LAUNCH_REAL(4, ODD); break;         \
+        case 1: TIP_LAUNCH_REAL_NM(1, ODD); break;       \
+        case 2: TIP_LAUNCH_REAL_NM(2, ODD); break;       \
+        case 8: TIP_LAUNCH_REAL_NM(8, ODD); break;       \
+        default: TIP_LAUNCH_REAL_NM(4, ODD); break;      \
     }
     if (tab.odd) {
         TIP_LAUNCH_REAL_BY(true)
@@ -130,6 +213,7 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
         TIP_LAUNCH_REAL_BY(false)
     }
 #undef TIP_LAUNCH_REAL_BY
+#undef TIP_LAUNCH_REAL_NM
 #undef TIP_LAUNCH_REAL
     TIP_HIP(hipGetLastError());
     return TIP_OK;

@@ -97,7 +97,7 @@ SIGNATURES = {
     "tip_lattice_download": (_int, [_vp, _int, _vp]),
     "tip_lattice_set_table": (_int, [_vp, _vp, _i64, _d, _d, _int]),
     "tip_lattice_set_zn": (_int, [_vp, _d, _d, _d]),
-    "tip_lattice_set_geometry": (_int, [_vp, _int, _int, _int]),
+    "tip_lattice_set_geometry": (_int, [_vp, _int, _int, _int, _int]),
     "tip_lattice_step": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
                                 C.POINTER(C.c_float)]),
     "tip_lattice_vacuum_count": (_int, [_vp, _d, C.POINTER(C.c_ulonglong)]),

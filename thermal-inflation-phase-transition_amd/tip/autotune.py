@@ -1,8 +1,9 @@
 """Launch-geometry autotuner for the step kernels.
 
-The knobs are the ones the kernels expose: rows of 64 z-sites per block (1, 2, 4, 8), the
-number of x planes a block marches through, the XCD-aware block remap, and (ensembles) the
-number of persistent blocks.  Each candidate is timed with HIP events on the object's own
+The knobs are the ones the kernels expose: for a lattice the kernel (LDS row tiles with 1, 2
+or 4 y rows per thread, or flat tiles of 1, 2, 4 or 8 rows of 64 z-sites per block), the
+number of x planes a block marches through and the XCD-aware block remap; for an ensemble the
+flat-tile rows, planes per work item, remap and the number of persistent blocks.  Each candidate is timed with HIP events on the object's own
 stream over a few steps; the fastest is applied and remembered in a JSON cache keyed by the
 workload signature, so later runs of the same workload skip the search.
 
@@ -38,10 +39,15 @@ def _store_cache(key: str, value: dict) -> None:
         pass  # a read-only home only costs the search next time
 
 
-def lattice_candidates() -> List[Tuple[int, int, int]]:
+def lattice_candidates(complex_field: bool = False) -> List[Tuple[int, int, int, int]]:
+    """(rows, planes, xcd, kernel): the LDS row-tile kernel (real field) with 1, 2 or 4 y rows
+    per thread, and the flat-tile kernel with 1, 2, 4 or 8 rows of 64 sites per block."""
     out = []
+    if not complex_field:
+        for ty, cx, xcd in itertools.product((1, 2, 4), (8, 16, 32), (1, 0)):
+            out.append((ty, cx, xcd, 1))
     for by, cx, xcd in itertools.product((1, 2, 4, 8), (4, 8, 16, 32), (1, 0)):
-        out.append((by, cx, xcd))
+        out.append((by, cx, xcd, 2))
     return out
 
 
@@ -62,7 +68,7 @@ def tune_lattice(lat, step_kw: dict, candidates: Optional[Iterable] = None, step
             lat.set_geometry(*hit["best"])
             return hit
     results = []
-    for cand in (candidates or lattice_candidates()):
+    for cand in (candidates or lattice_candidates(lat.complex)):
         lat.set_geometry(*cand)
         lat.step(2, timed=True, **step_kw)
         ms = lat.step(steps, timed=True, **step_kw)

@@ -19,6 +19,9 @@ from ._lib import StepParams
 REAL, COMPLEX = 0, 1
 
 
+KERNEL_AUTO, KERNEL_ROWS, KERNEL_FLAT = 0, 1, 2
+
+
 class Lattice:
     def __init__(self, shape, dtype=np.float32, complex_field: bool = False):
         self.shape = tuple(int(s) for s in shape)
@@ -65,11 +68,14 @@ class Lattice:
     def set_zn(self, n: float, dv: float, active: float) -> None:
         _lib.call("tip_lattice_set_zn", self._h, float(n), float(dv), float(active))
 
-    def set_geometry(self, rows_per_block: int = 0, planes_per_block: int = 0,
-                     xcd_remap: bool = True) -> None:
-        """Launch geometry of the float32 step kernel (0 = default)."""
-        _lib.call("tip_lattice_set_geometry", self._h, int(rows_per_block), int(planes_per_block),
-                  int(bool(xcd_remap)))
+    def set_geometry(self, rows: int = 0, planes_per_block: int = 0, xcd_remap: bool = True,
+                     kernel: int = KERNEL_AUTO) -> None:
+        """Launch geometry of the float32 step kernel (0 = default).  kernel KERNEL_ROWS: `rows`
+        = y rows per thread of the LDS row-tile kernel (real field); KERNEL_FLAT: `rows` = rows
+        of 64 z-sites per block of the flat-tile kernel; KERNEL_AUTO: row tiles where they
+        apply."""
+        _lib.call("tip_lattice_set_geometry", self._h, int(kernel), int(rows),
+                  int(planes_per_block), int(bool(xcd_remap)))
 
     # -- evolution -----------------------------------------------------------------------
     def step(self, nsteps: int, *, dt: float, dx: float, eta_eff: float, mu: float,
