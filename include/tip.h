@@ -272,10 +272,12 @@ int tip_ensemble_download(tip_ensemble *ens, int inst, int which, float *host);
 int tip_ensemble_set_table(tip_ensemble *ens, int inst, const double *host_table, int64_t npts,
                            double tmin, double dinv, int odd);
 int tip_ensemble_set_params(tip_ensemble *ens, int inst, const tip_step_params *p);
-/* rows of 64 z-sites per block (1, 2, 4, 8), x planes per work item, XCD-aware block remap,
- * and persistent_blocks > 0 to launch that many blocks each walking a contiguous item range
- * (0 = one block per item). */
-int tip_ensemble_set_geometry(tip_ensemble *ens, int rows_per_block, int planes_per_block,
+/* kernel TIP_KERNEL_ROWS (LDS row tiles, `rows` = y rows per thread, 1, 2 or 4 dividing every
+ * instance's ny, 0 = default; needs every nz <= 256), TIP_KERNEL_FLAT (`rows` = rows of 64
+ * z-sites per block, 1, 2, 4 or 8) or TIP_KERNEL_AUTO (row tiles where they apply); x planes
+ * per work item, XCD-aware block remap, and persistent_blocks > 0 to launch that many blocks
+ * each walking a contiguous item range (0 = one block per item). */
+int tip_ensemble_set_geometry(tip_ensemble *ens, int kernel, int rows, int planes_per_block,
                               int xcd_remap, int persistent_blocks);
 int tip_ensemble_step(tip_ensemble *ens, int64_t nsteps, int mode, float *elapsed_ms);
 int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long *host_counts);

@@ -69,7 +69,11 @@ def test_modes_match_single_lattice(mode):
     ens.close()
 
 
-@pytest.mark.parametrize("geometry", [(1, 3, 0, 0), (2, 16, 1, 0), (8, 4, 1, 16), (4, 8, 1, 40)])
+# (rows, planes, xcd, persistent blocks, kernel): flat tiles (2) and LDS row tiles (1); the
+# default (auto) is row tiles with one y row per thread here (ny = 33 is odd)
+@pytest.mark.parametrize("geometry", [(1, 3, 0, 0, 2), (2, 16, 1, 0, 2), (8, 4, 1, 16, 2),
+                                      (4, 8, 1, 40, 2), (1, 3, 0, 0, 1), (1, 16, 1, 7, 1),
+                                      (0, 5, 1, 0, 1), (1, 1, 0, 3, 1)])
 def test_geometries_identical(geometry):
     ens_a, _, _, _ = _setup()
     ens_a.step(5, mode="batched")

@@ -25,6 +25,7 @@ struct EnsInst {
     float tmin, dinv, off;
     int last, odd;
     int nx, ny, nz;
+    int W;  // waves per z row in the row-tile kernel (1, 2 or 4)
     uint64_t seed_mix;
     int hash;
 };
@@ -68,6 +69,41 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_ensemble(
     }
 }
 
+// Row-tile variant (LDS staging, y rows per thread TY): a block is 4 waves; an instance with
+// W waves per z row gets 4/W row groups per block.
+template <int TY, bool ODD>
+__global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
+    const EnsInst* __restrict__ insts, const EnsItem* __restrict__ items, int nitems, int cx,
+    int parity, const uint64_t* __restrict__ step_base, int step_off, int xcd_remap,
+    unsigned long long* __restrict__ vac, float esc)
+{
+    extern __shared__ float lds_ens[];
+    int b = (int)blockIdx.x;
+    const int nb = (int)gridDim.x;
+    if (xcd_remap) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int q = (nitems + nb - 1) / nb;
+    const int i0 = b * q;
+    const int i1 = min(i0 + q, nitems);
+    const uint64_t step = step_base[0] + (uint64_t)step_off;
+    for (int i = i0; i < i1; ++i) {
+        const EnsItem it = items[i];
+        const EnsInst& I = insts[it.inst];
+        TableF32<ODD> tab;
+        tab.ab = I.ab;
+        tab.tmin = I.tmin;
+        tab.dinv = I.dinv;
+        tab.last = I.last;
+        tab.off = I.off;
+        const int x1 = min(it.x0 + cx, I.nx);
+        __syncthreads();  // the previous item's last plane may still be read from LDS
+        unsigned cnt = rk2_real_rows_tile<ODD, TY, NOISE_HASH>(
+            I.f[parity][0], I.f[parity][1], I.f[parity ^ 1][0], I.f[parity ^ 1][1], nullptr, I.c,
+            tab, I.nx, I.ny, I.nz, I.W, it.tp, it.x0, x1, step ^ I.seed_mix, vac != nullptr, esc,
+            lds_ens);
+        if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
+    }
+}
+
 __global__ void k_advance(uint64_t* step_base, int n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) step_base[0] += (uint64_t)n;
@@ -94,8 +130,10 @@ struct tip_ensemble {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* dev_vac = nullptr;
-    // geometry
-    int by = 4, cx = 8, xcd = 1, persistent = 0;
+    // geometry: kind (TIP_KERNEL_*), by = rows (flat: 64-site rows per block; row tiles: y rows
+    // per thread, 0 = default), cx = x planes per work item
+    int kind = TIP_KERNEL_AUTO, by = 0, cx = 8, xcd = 1, persistent = 0;
+    int use_rows = 0, ty = 1, flat_by = 4;  // resolved by build_items
     EnsItem* dev_items = nullptr;
     int nitems = 0;
     bool items_dirty = true;
@@ -166,16 +204,51 @@ int sync_descriptors(tip_ensemble* E)
     return TIP_OK;
 }
 
+// Whether the row-tile kernel runs this ensemble and with how many rows per thread.
+int resolve_kernel(tip_ensemble* E)
+{
+    bool rows_ok = E->kind != TIP_KERNEL_FLAT;
+    int ty = E->by;
+    if (E->kind == TIP_KERNEL_FLAT || (E->kind == TIP_KERNEL_AUTO && E->by == 8)) rows_ok = false;
+    if (rows_ok && ty == 0) {  // default: the largest of 4, 2, 1 dividing every ny
+        ty = 4;
+        for (int i = 0; i < E->n; ++i)
+            while (E->dims[3 * i + 1] % ty) ty >>= 1;
+    }
+    for (int i = 0; rows_ok && i < E->n; ++i)
+        if (E->dims[3 * i + 2] > 256 || E->dims[3 * i + 1] % ty) rows_ok = false;
+    TIP_REQUIRE(rows_ok || E->kind != TIP_KERNEL_ROWS,
+                "ensemble: the row kernel needs nz <= 256 and rows per thread dividing every ny");
+    E->use_rows = rows_ok ? 1 : 0;
+    E->ty = rows_ok ? ty : 1;
+    E->flat_by = (!rows_ok && E->by) ? E->by : 4;
+    return TIP_OK;
+}
+
 int build_items(tip_ensemble* E)
 {
     if (!E->items_dirty) return TIP_OK;
+    int rc = resolve_kernel(E);
+    if (rc) return rc;
     std::vector<EnsItem> items;
     for (int i = 0; i < E->n; ++i) {
         const int nx = (int)E->dims[3 * i], ny = (int)E->dims[3 * i + 1], nz = (int)E->dims[3 * i + 2];
-        const int tiles = (ny * nz + 64 * E->by - 1) / (64 * E->by);
+        int tiles;
+        if (E->use_rows) {
+            int W = (nz + 63) / 64;
+            if (W == 3) W = 4;
+            E->host_inst[i].W = W;
+            const int G = 4 / W;
+            tiles = (ny + G * E->ty - 1) / (G * E->ty);
+        } else {
+            tiles = (ny * nz + 64 * E->flat_by - 1) / (64 * E->flat_by);
+        }
         for (int x0 = 0; x0 < nx; x0 += E->cx)
             for (int t = 0; t < tiles; ++t) items.push_back({i, t, x0, 0});
     }
+    E->desc_dirty = true;
+    rc = sync_descriptors(E);
+    if (rc) return rc;
     TIP_REQUIRE(!items.empty() && items.size() < (size_t(1) << 30), "ensemble: bad work list");
     TIP_HIP(hipStreamSynchronize(E->stream));
     if (E->dev_items) TIP_HIP(hipFree(E->dev_items));
@@ -194,12 +267,34 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
     int nb = E->nitems;
     if (E->persistent > 0 && E->persistent < nb) nb = E->persistent;
     const int remap = (E->xcd && nb % 8 == 0) ? 1 : 0;
+    if (E->use_rows) {
+        const size_t lds = (size_t)256 * (4 * E->ty + 1) * sizeof(float);
+#define TIP_LAUNCH_ENSR(TYV, ODD)                                                                \
+    hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD>), dim3(nb), dim3(64, 4), lds, E->stream, \
+                       E->dev_inst, E->dev_items, E->nitems, E->cx, parity, E->dev_step,        \
+                       step_off, remap, vac, esc)
+#define TIP_LAUNCH_ENSR_TY(ODD)                        \
+    switch (E->ty) {                                   \
+        case 4: TIP_LAUNCH_ENSR(4, ODD); break;        \
+        case 2: TIP_LAUNCH_ENSR(2, ODD); break;        \
+        default: TIP_LAUNCH_ENSR(1, ODD); break;       \
+    }
+        if (E->host_inst[0].odd) {
+            TIP_LAUNCH_ENSR_TY(true)
+        } else {
+            TIP_LAUNCH_ENSR_TY(false)
+        }
+#undef TIP_LAUNCH_ENSR_TY
+#undef TIP_LAUNCH_ENSR
+        TIP_HIP(hipGetLastError());
+        return TIP_OK;
+    }
 #define TIP_LAUNCH_ENS(BYV, ODD)                                                                \
     hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, E->stream,     \
                        E->dev_inst, E->dev_items, E->nitems, E->cx, parity, E->dev_step,       \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENS_BY(ODD)                         \
-    switch (E->by) {                                   \
+    switch (E->flat_by) {                              \
         case 1: TIP_LAUNCH_ENS(1, ODD); break;         \
         case 2: TIP_LAUNCH_ENS(2, ODD); break;         \
         case 8: TIP_LAUNCH_ENS(8, ODD); break;         \
@@ -348,18 +443,31 @@ extern "C" int tip_ensemble_set_params(tip_ensemble* E, int i, const tip_step_pa
     return TIP_OK;
 }
 
-extern "C" int tip_ensemble_set_geometry(tip_ensemble* E, int rows_per_block,
+extern "C" int tip_ensemble_set_geometry(tip_ensemble* E, int kernel, int rows,
                                          int planes_per_block, int xcd_remap,
                                          int persistent_blocks)
 {
     TIP_REQUIRE(E, "tip_ensemble_set_geometry: NULL");
-    TIP_REQUIRE(rows_per_block == 1 || rows_per_block == 2 || rows_per_block == 4 ||
-                    rows_per_block == 8,
-                "ensemble geometry: rows per block must be 1, 2, 4 or 8");
+    TIP_REQUIRE(kernel == TIP_KERNEL_AUTO || kernel == TIP_KERNEL_ROWS ||
+                    kernel == TIP_KERNEL_FLAT,
+                "ensemble geometry: unknown kernel %d", kernel);
+    TIP_REQUIRE(kernel == TIP_KERNEL_ROWS ? (rows == 0 || rows == 1 || rows == 2 || rows == 4)
+                : kernel == TIP_KERNEL_FLAT
+                    ? (rows == 1 || rows == 2 || rows == 4 || rows == 8)
+                    : (rows == 0 || rows == 1 || rows == 2 || rows == 4 || rows == 8),
+                "ensemble geometry: rows %d not valid for kernel %d", rows, kernel);
     TIP_REQUIRE(planes_per_block >= 1 && planes_per_block <= 4096,
                 "ensemble geometry: planes per block out of range");
     TIP_REQUIRE(persistent_blocks >= 0, "ensemble geometry: persistent_blocks < 0");
-    E->by = rows_per_block;
+    const int old_kind = E->kind, old_by = E->by;
+    E->kind = kernel;
+    E->by = rows;
+    int rc = resolve_kernel(E);
+    if (rc) {
+        E->kind = old_kind;
+        E->by = old_by;
+        return rc;
+    }
     E->cx = planes_per_block;
     E->xcd = xcd_remap ? 1 : 0;
     E->persistent = persistent_blocks;

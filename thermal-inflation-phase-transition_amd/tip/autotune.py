@@ -51,17 +51,20 @@ def lattice_candidates(complex_field: bool = False) -> List[Tuple[int, int, int,
     return out
 
 
-def ensemble_candidates() -> List[Tuple[int, int, int, int]]:
+def ensemble_candidates() -> List[Tuple[int, int, int, int, int]]:
+    """(rows, planes, xcd, persistent, kernel) over row tiles (kernel 1) and flat tiles (2)."""
     out = []
-    for by, cx, pers in itertools.product((2, 4, 8), (4, 8, 16, 32), (0, 2048, 4096)):
-        out.append((by, cx, 1, pers))
+    for ty, cx, pers in itertools.product((1, 2, 4), (8, 16, 32), (0, 2048)):
+        out.append((ty, cx, 1, pers, 1))
+    for by, cx, pers in itertools.product((2, 4, 8), (8, 16, 32), (0, 2048)):
+        out.append((by, cx, 1, pers, 2))
     return out
 
 
 def tune_lattice(lat, step_kw: dict, candidates: Optional[Iterable] = None, steps: int = 20,
                  use_cache: bool = True) -> dict:
     """Time each (rows, planes, xcd) geometry on a Lattice and apply the fastest."""
-    key = f"lattice:{lat.shape}:{lat.dtype.name}:{'c' if lat.complex else 'r'}"
+    key = f"lattice2:{lat.shape}:{lat.dtype.name}:{'c' if lat.complex else 'r'}"
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
@@ -85,7 +88,7 @@ def tune_lattice(lat, step_kw: dict, candidates: Optional[Iterable] = None, step
 def tune_ensemble(ens, candidates: Optional[Iterable] = None, steps: int = 10,
                   mode: str = "graph", use_cache: bool = True) -> dict:
     """Time each (rows, planes, xcd, persistent) geometry on an Ensemble and apply the best."""
-    key = f"ensemble:{sorted(set(ens.shapes))}:{len(ens.shapes)}:{mode}"
+    key = f"ensemble2:{sorted(set(ens.shapes))}:{len(ens.shapes)}:{mode}"
     if use_cache:
         hit = _load_cache().get(key)
         if hit:

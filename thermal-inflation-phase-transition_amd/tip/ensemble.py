@@ -50,9 +50,11 @@ class Ensemble:
         p = StepParams(dt, dx, eta_eff, mu, inv_a2, noise_scale)
         _lib.call("tip_ensemble_set_params", self._h, i, C.byref(p))
 
-    def set_geometry(self, rows_per_block=4, planes_per_block=8, xcd_remap=True,
-                     persistent_blocks=0) -> None:
-        _lib.call("tip_ensemble_set_geometry", self._h, int(rows_per_block),
+    def set_geometry(self, rows=0, planes_per_block=8, xcd_remap=True, persistent_blocks=0,
+                     kernel=0) -> None:
+        """kernel 0 = auto (LDS row tiles where every nz <= 256), 1 = row tiles (`rows` = y rows
+        per thread), 2 = flat tiles (`rows` = rows of 64 sites per block)."""
+        _lib.call("tip_ensemble_set_geometry", self._h, int(kernel), int(rows),
                   int(planes_per_block), int(bool(xcd_remap)), int(persistent_blocks))
 
     def step(self, nsteps: int, mode: str = "graph", timed: bool = False):
