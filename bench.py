@@ -154,6 +154,87 @@ def bench_ensemble(steps: int = 200):
     return out
 
 
+def bench_extras(steps: int = 200) -> dict:
+    """The other forms on one GPU: complex 256^3 float32 step and winding (config 2 shape),
+    the float64 drop-in real 256^3 step, and the complex 512^3 step with the inline MLP
+    surrogate (config 4 shape; throughput does not depend on the weights, untrained ones are
+    used)."""
+    from tip import potential as P
+    from tip import surrogate as S
+    from tip.lattice import Lattice
+
+    out = {}
+    mu, T0 = 1000.0, 6770.0
+    n = 256
+    pc = P.ModelParams.complex()
+    vt = P.thermal_vev(T0, pc)
+    t, tmin, dinv = P.build_vprime_table(T0, 0.0, 1.5 * vt, 8192, pc)
+    dt, dx, eta = mu * 2.5e-4, mu * 1e-3, 1000.0 / mu
+    kw = dict(dt=dt, dx=dx, eta_eff=eta, mu=mu, inv_a2=1.0,
+              noise_scale=P.noise_scale(eta, T0, dt, mu, 1e-3))
+    shape = (n, n, n)
+    rng = np.random.default_rng(2)
+    th = 2 * np.pi * rng.random(shape)
+    r = 0.01 * np.abs(rng.normal(size=shape))
+    lat = Lattice(shape, np.float32, complex_field=True)
+    lat.upload("phi1", r * np.cos(th))
+    lat.upload("phi2", r * np.sin(th))
+    lat.upload("pi1", np.zeros(shape))
+    lat.upload("pi2", np.zeros(shape))
+    lat.set_table(t, tmin, dinv, odd=False)
+    lat.step(10, timed=True, **kw)
+    ms = lat.step(steps, timed=True, **kw)
+    us = 1e3 * ms / steps
+    t0 = time.perf_counter()
+    for _ in range(5):
+        strings, plaq = lat.winding_counts()
+    w_ms = 1e3 * (time.perf_counter() - t0) / 5
+    out["complex256"] = {"us_per_step": us, "site_updates_per_s": n ** 3 / (us * 1e-6),
+                         "hbm_gbps_algorithmic": 32.0 * n ** 3 / (us * 1e-6) / 1e9,
+                         "bytes_per_site": 32.0, "winding_ms_wall": w_ms,
+                         "winding_gbps_algorithmic": 11.0 * n ** 3 / (w_ms * 1e-3) / 1e9,
+                         "strings": strings}
+    lat.close()
+    log(f"complex 256^3: {us:.1f} us/step, winding {w_ms:.2f} ms")
+
+    table, tmin, dinv, prm = real_workload(n)
+    lat = Lattice(shape, np.float64)
+    lat.upload("phi", 0.01 * np.random.default_rng(0).normal(size=shape))
+    lat.upload("pi", np.zeros(shape))
+    lat.set_table(table, tmin, dinv, odd=True)
+    kw64 = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+                noise_scale=prm["noise_scale"])
+    lat.step(5, timed=True, **kw64)
+    ms = lat.step(50, timed=True, **kw64)
+    us = 1e3 * ms / 50
+    out["real256_f64"] = {"us_per_step": us, "site_updates_per_s": n ** 3 / (us * 1e-6),
+                          "hbm_gbps_algorithmic": 32.0 * n ** 3 / (us * 1e-6) / 1e9,
+                          "bytes_per_site": 32.0}
+    lat.close()
+    log(f"real 256^3 float64: {us:.1f} us/step")
+
+    m = 512
+    shape = (m, m, m)
+    sur = S.Surrogate(S.SurrogateSpec(), seed=0)
+    lat = Lattice(shape, np.float32, complex_field=True)
+    for k in ("phi1", "phi2", "pi1", "pi2"):
+        lat.upload(k, np.zeros(shape, np.float32) + (1.0 if k == "phi1" else 0.0))
+    kw4 = dict(dt=dt, dx=dx, eta_eff=eta, mu=mu, noise_scale=kw["noise_scale"])
+    dT = (6770.0 - 7350.0) / 1e5
+    lat.step_mlp(2, sur, T_start=7350.0, dT_per_step=dT, timed=True, **kw4)
+    nst = 10
+    ms = lat.step_mlp(nst, sur, T_start=7350.0, dT_per_step=dT, timed=True, **kw4)
+    us = 1e3 * ms / nst
+    flop = 2 * 2 * (2 * 32 + 32 * 32 + 32) * m ** 3  # two RK2 stages, FMA = 2 flop
+    out["surrogate512"] = {"us_per_step": us, "site_updates_per_s": m ** 3 / (us * 1e-6),
+                           "mlp_tflops": flop / (us * 1e-6) / 1e12,
+                           "flop_per_site_step": flop / m ** 3,
+                           "hbm_gbps_algorithmic": 32.0 * m ** 3 / (us * 1e-6) / 1e9}
+    lat.close()
+    log(f"complex 512^3 with the inline MLP: {us:.1f} us/step")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +243,8 @@ def main() -> int:
     ap.add_argument("--workload", default="real256", choices=["real256", "real256_f64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the complex, float64 and surrogate throughput lines")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, 5 device-to-device copies of 128 MiB (a known-size "
@@ -249,6 +332,8 @@ def main() -> int:
     if not a.no_ensemble:
         log("batched ensemble (64 realisations, 46.4M sites) ...")
         out["ensemble"] = bench_ensemble()
+    if not a.no_extras:
+        out["extras"] = bench_extras()
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)

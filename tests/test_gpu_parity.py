@@ -148,6 +148,56 @@ def test_f32_error_against_f64():
     assert e_phi < 1e-4 and e_pi < 1e-4
 
 
+def test_f32_long_run_against_f64_ensemble():
+    """Config-0 shape over 10^4 steps: the float32 path against the float64 path (itself within
+    1e-12 of the oracle) for 4 seeds.  Past the first few hundred steps the fields of a chaotic
+    Langevin run decorrelate whatever the precision, so the bar is on the true-vacuum fraction
+    series: ensemble means within the ensembles' combined 1 sigma (plus one site)."""
+    from tip.lattice import Lattice
+    shape = (64, 64, 64)
+    prm = c0_params()
+    t, tmin, dinv = c0_table()
+    kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+              noise_scale=prm["noise_scale"])
+    marks = list(range(100, 2001, 100)) + list(range(3000, 10001, 1000))
+    series = {np.float32: [], np.float64: []}
+    err100, err_end = [], []
+    for seed in range(4):
+        phi0 = 0.01 * np.random.default_rng(100 + seed).normal(size=shape)
+        lats = {}
+        for dt_ in (np.float32, np.float64):
+            lat = Lattice(shape, dt_)
+            lat.upload("phi", phi0)
+            lat.upload("pi", np.zeros(shape))
+            lat.set_table(t, tmin, dinv, odd=True)
+            lats[dt_] = lat
+        fs = {np.float32: [], np.float64: []}
+        done = 0
+        for mk in marks:
+            for dt_, lat in lats.items():
+                lat.step(mk - done, seed=seed, step0=done, **kw)
+                fs[dt_].append(lat.vacuum_fraction(1e4))
+            if mk == 100:
+                a = lats[np.float64].download("phi")
+                err100.append(rel(lats[np.float32].download("phi").astype(np.float64), a))
+            done = mk
+        a = lats[np.float64].download("phi")
+        err_end.append(rel(lats[np.float32].download("phi").astype(np.float64), a))
+        for dt_, lat in lats.items():
+            series[dt_].append(fs[dt_])
+            lat.close()
+    f32, f64 = np.array(series[np.float32]), np.array(series[np.float64])
+    m32, m64 = f32.mean(0), f64.mean(0)
+    s32, s64 = f32.std(0, ddof=1), f64.std(0, ddof=1)
+    print(f"[f32 vs f64] field rel err after 100 steps {np.max(err100):.3e}, after 10^4 "
+          f"{np.max(err_end):.3e}")
+    for mk, a, b, c, d in zip(marks, m32, s32, m64, s64):
+        print(f"  step {mk:6d}: f_true f32 {a:.5f} +- {b:.5f}   f64 {c:.5f} +- {d:.5f}")
+    assert np.max(err100) < 1e-4
+    tol = np.sqrt(s32 ** 2 + s64 ** 2) + 1.0 / np.prod(shape)
+    assert np.all(np.abs(m32 - m64) <= tol)
+
+
 def test_f32_matches_oracle_f32mixed_one_step():
     """float32 kernel vs the oracle's float32-mixed semantics, one step, explicit noise."""
     tip, K, _, DA, _ = _tip()
