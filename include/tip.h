@@ -229,10 +229,10 @@ int tip_lattice_set_table(tip_lattice *lat, const double *host_table, int64_t np
                           double tmin, double dinv, int odd);
 int tip_lattice_set_zn(tip_lattice *lat, double zn_n, double zn_dv, double zn_active);
 /* Launch geometry of the float32 step kernels (the autotuner's knobs).  kernel:
- * TIP_KERNEL_AUTO (row tiles where they apply, else flat tiles), TIP_KERNEL_ROWS (real field:
- * whole z rows staged in LDS, `rows` = y rows per thread, 1, 2 or 4, dividing ny; needs
- * nz <= 1024) or TIP_KERNEL_FLAT (`rows` = rows of 64 z-sites per block, 1, 2, 4 or 8; the
- * complex field always uses flat tiles).  rows = 0 picks the default, planes_per_block = x
+ * TIP_KERNEL_AUTO (row tiles where they apply, else flat tiles), TIP_KERNEL_ROWS (whole z
+ * rows staged in LDS, `rows` = y rows per thread, 1, 2 or 4, dividing ny; needs nz <= 1024) or
+ * TIP_KERNEL_FLAT (`rows` = rows of 64 z-sites per block, 1, 2, 4 or 8).  float32 lattices;
+ * the float64 drop-in has one geometry.  rows = 0 picks the default, planes_per_block = x
  * planes each block marches through (0 = derived), xcd_remap = XCD-aware block order (0/1). */
 #define TIP_KERNEL_AUTO 0
 #define TIP_KERNEL_ROWS 1

@@ -739,6 +739,242 @@ __global__ __launch_bounds__(1024) void k_rk2_real_rows(
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
+// ---------------------------------------------------------------------------------------
+// Complex float32 row-tile kernel (LDS staging; default for the complex field).  Same tile,
+// window and LDS scheme as the real row kernel with four fields (phi1, phi2, pi1, pi2): 4 + 8/TY
+// global loads per site.  Force: rho = |phi|, V'(rho) from the (even) segment table,
+// dV_i = V'(rho) phi_i / max(rho, 1e-30) (latticeSimComplex_numba.py:2130-2146), Z_N term
+// (:2147-2152) for integer orders as Im((phi1 + i phi2)^N) / rho^N, otherwise sin(N atan2).
+// Noise: (cos, sin) of one hash draw per site, as the reference's complex kernels.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void gauss_pair_f32_l2(uint64_t seed, uint32_t word_mask, float& zc,
+                                                  float& zs)
+{
+    uint64_t x1, x2;
+    hash_states(seed, x1, x2);
+    const float u1 = fmaxf(unit_f32(x1, word_mask), 1e-12f);
+    const float u2 = (float)((uint32_t)(x2 >> 32) & word_mask) * 0x1p-32f;
+    const float r = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+    zc = r * __builtin_amdgcn_cosf(u2);
+    zs = r * __builtin_amdgcn_sinf(u2);
+}
+
+__device__ __forceinline__ void complex_force_f32(__amdgpu_buffer_rsrc_t T,
+                                                  const TableF32<false>& tab,
+                                                  const ZnParams<float>& zn, float a, float b,
+                                                  float& d1, float& d2)
+{
+    const float rho = __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
+    const float inv = __builtin_amdgcn_rcpf(fmaxf(rho, 1e-30f));
+    const float w = seg_eval<false>(seg_load<false>(T, tab, rho), rho) * inv;
+    d1 = w * a;
+    d2 = w * b;
+    if (zn.active && rho > 1e-20f) {
+        float sn;
+        if (zn.ipow > 0) {
+            const float ca = a * inv, sb = b * inv;
+            float re = 1.0f, im = 0.0f;
+            for (int e = 0; e < zn.ipow; ++e) {
+                const float t = fmaf(re, ca, -im * sb);
+                im = fmaf(re, sb, im * ca);
+                re = t;
+            }
+            sn = im;
+        } else {
+            sn = sinf(zn.n * atan2f(b, a));
+        }
+        const float cz = zn.n * zn.dv * sn * inv * inv;
+        d1 = fmaf(-cz, b, d1);
+        d2 = fmaf(cz, a, d2);
+    }
+}
+
+template <int TY, int NM>
+__device__ __forceinline__ unsigned rk2_complex_rows_tile(
+    const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
+    const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
+    float* __restrict__ o3, float* __restrict__ o4, const float* __restrict__ noise1,
+    const float* __restrict__ noise2, const StepScalars<float>& c, const TableF32<false>& tab,
+    const ZnParams<float>& zn, int nx, int ny, int nz, int W, int tile, int x0, int x1,
+    uint64_t noise_seed, bool count, float esc2, float* __restrict__ lds)
+{
+    constexpr int NF = 4;                 // phi1, phi2, pi1, pi2
+    constexpr int K = 2 * NF * TY + 1;    // LDS words per z column
+    const int G = (int)blockDim.y / W;
+    const int w = (int)threadIdx.y, grp = w / W, zw = w - grp * W;
+    const int z = zw * 64 + (int)threadIdx.x;
+    const bool zact = z < nz;
+    const int zc = zact ? z : 0;
+    const int zp = zc + 1 < nz ? zc + 1 : 0, zm = zc > 0 ? zc - 1 : nz - 1;
+    const int y0 = (tile * G + grp) * TY;
+    const bool act = zact && y0 < ny;
+    const int yb = y0 < ny ? y0 : 0;
+    float* Lg = lds + (size_t)grp * (64 * W) * K;
+    float* Lz = Lg + z * K;
+    const float* Lzp = Lg + zp * K;
+    const float* Lzm = Lg + zm * K;
+
+    const int nyz = ny * nz;
+    const size_t plane = (size_t)nyz * 4u;
+    const uint32_t pb = (uint32_t)plane;
+    const char* F[NF] = {reinterpret_cast<const char*>(phi1), reinterpret_cast<const char*>(phi2),
+                         reinterpret_cast<const char*>(pi1), reinterpret_cast<const char*>(pi2)};
+    char* O[NF] = {reinterpret_cast<char*>(o1), reinterpret_cast<char*>(o2),
+                   reinterpret_cast<char*>(o3), reinterpret_cast<char*>(o4)};
+    auto at = [&](const char* bp, int x) { return make_rsrc(bp + (size_t)x * plane, pb); };
+    const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
+    uint32_t off[TY], offs[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        off[r] = (uint32_t)((yb + r) * nz + zc) * 4u;
+        offs[r] = act ? off[r] : 0xFFFFFFF0u;
+    }
+    const uint32_t offm = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz + zc) * 4u;
+    const uint32_t offp = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz + zc) * 4u;
+    const uint64_t planeK = (uint64_t)(uint32_t)nyz * (uint64_t)kNoiseMult;
+    const uint64_t col0K = (uint64_t)(uint32_t)(yb * nz + zc) * (uint64_t)kNoiseMult;
+    const uint64_t rowK = (uint64_t)(uint32_t)nz * (uint64_t)kNoiseMult;
+    uint64_t xK = (uint64_t)(uint32_t)x0 * planeK;
+
+    float v0[NF][TY], v1[NF][TY], v2[NF][TY], v3[NF][TY];
+    {
+        const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
+        const int xq = x0 + 1 < nx ? x0 + 1 : 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                v0[f][r] = buf_load<float>(at(F[f], xm), off[r]);
+                v1[f][r] = buf_load<float>(at(F[f], x0), off[r]);
+                v2[f][r] = buf_load<float>(at(F[f], xq), off[r]);
+            }
+    }
+    float hA[2 * NF], hB[2 * NF];  // [f] row y0-1, [NF + f] row y0+TY
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        hA[f] = buf_load<float>(at(F[f], x0), offm);
+        hA[NF + f] = buf_load<float>(at(F[f], x0), offp);
+    }
+    unsigned cnt = 0;
+
+    auto plane_step = [&](auto bufc, int x, const float (&vm)[NF][TY], const float (&vc)[NF][TY],
+                          const float (&vp)[NF][TY], float (&vn)[NF][TY],
+                          const float (&hc)[2 * NF], float (&hn)[2 * NF]) {
+        constexpr int b = decltype(bufc)::value;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < TY; ++r) Lz[(b * NF + f) * TY + r] = vc[f][r];
+        {
+            const int xa = x + 1 < nx ? x + 1 : 0;
+            const int xb = xa + 1 < nx ? xa + 1 : 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                for (int r = 0; r < TY; ++r) vn[f][r] = buf_load<float>(at(F[f], xb), off[r]);
+                hn[f] = buf_load<float>(at(F[f], xa), offm);
+                hn[NF + f] = buf_load<float>(at(F[f], xa), offp);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            float sum[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const float yp = r + 1 < TY ? vc[f][r + 1 < TY ? r + 1 : r] : hc[NF + f];
+                const float ym = r > 0 ? vc[f][r > 0 ? r - 1 : 0] : hc[f];
+                const float zpv = Lzp[(b * NF + f) * TY + r], zmv = Lzm[(b * NF + f) * TY + r];
+                sum[f] = ((((vp[f][r] + vm[f][r]) + yp) + ym) + zpv) + zmv;
+            }
+            const float a = vc[0][r], bb = vc[1][r], u = vc[2][r], v = vc[3][r];
+            const float t1a = fmaf(-6.0f, a, sum[0]), t1b = fmaf(-6.0f, bb, sum[1]);
+            const float fa = fmaf(c.half_dt, u, a), fb = fmaf(c.half_dt, v, bb);
+            float n1 = 0.0f, n2 = 0.0f;
+            if (NM == NOISE_ARRAY) {
+                n1 = buf_load<float>(at(reinterpret_cast<const char*>(noise1), x), off[r]);
+                if (noise2) n2 = buf_load<float>(at(reinterpret_cast<const char*>(noise2), x), off[r]);
+            }
+            float g1 = 0.0f, g2 = 0.0f;
+            if (NM == NOISE_HASH)
+                gauss_pair_f32_l2((col0K + (uint64_t)r * rowK + xK) ^ noise_seed, c.word_mask, g1,
+                                  g2);
+            float d1, d2;
+            complex_force_f32(T, tab, zn, a, bb, d1, d2);
+            const float k1a = fmaf(-c.inv_mu2, d1, fmaf(c.c_lap, t1a, -c.eta * u));
+            const float k1b = fmaf(-c.inv_mu2, d2, fmaf(c.c_lap, t1b, -c.eta * v));
+            const float ga = fmaf(c.half_dt, k1a, u), gb = fmaf(c.half_dt, k1b, v);
+            const float t2a = fmaf(c.half_dt, fmaf(-6.0f, u, sum[2]), t1a);
+            const float t2b = fmaf(c.half_dt, fmaf(-6.0f, v, sum[3]), t1b);
+            float e1, e2;
+            complex_force_f32(T, tab, zn, fa, fb, e1, e2);
+            const float k2a = fmaf(-c.inv_mu2, e1, fmaf(c.c_lap, t2a, -c.eta * ga));
+            const float k2b = fmaf(-c.inv_mu2, e2, fmaf(c.c_lap, t2b, -c.eta * gb));
+            const float na = fmaf(c.dt, ga, a), nb = fmaf(c.dt, gb, bb);
+            float pa, pbv;
+            if (NM == NOISE_HASH) {
+                pa = fmaf(c.dt, k2a, fmaf(c.ns_l2, g1, u));
+                pbv = fmaf(c.dt, k2b, fmaf(c.ns_l2, g2, v));
+            } else if (NM == NOISE_ARRAY) {
+                pa = u + fmaf(c.dt, k2a, n1);
+                pbv = v + fmaf(c.dt, k2b, n2);
+            } else {
+                pa = fmaf(c.dt, k2a, u);
+                pbv = fmaf(c.dt, k2b, v);
+            }
+            buf_store(na, at(O[0], x), offs[r]);
+            buf_store(nb, at(O[1], x), offs[r]);
+            buf_store(pa, at(O[2], x), offs[r]);
+            buf_store(pbv, at(O[3], x), offs[r]);
+            if (count) cnt += wave_count(act && fmaf(na, na, nb * nb) > esc2);
+        }
+        xK += planeK;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    int x = x0;
+    for (; x + 4 <= x1; x += 4) {
+        plane_step(B0{}, x + 0, v0, v1, v2, v3, hA, hB);
+        plane_step(B1{}, x + 1, v1, v2, v3, v0, hB, hA);
+        plane_step(B0{}, x + 2, v2, v3, v0, v1, hA, hB);
+        plane_step(B1{}, x + 3, v3, v0, v1, v2, hB, hA);
+    }
+    auto rotate = [&]() {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                v0[f][r] = v1[f][r]; v1[f][r] = v2[f][r]; v2[f][r] = v3[f][r];
+            }
+#pragma unroll
+        for (int k = 0; k < 2 * NF; ++k) hA[k] = hB[k];
+    };
+    if (x < x1) { plane_step(B0{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
+    if (x < x1) { plane_step(B1{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
+    if (x < x1) plane_step(B0{}, x, v0, v1, v2, v3, hA, hB);
+    return cnt;
+}
+
+template <int TY, int NM>
+__global__ __launch_bounds__(512) void k_rk2_complex_rows(
+    const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
+    const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
+    float* __restrict__ o3, float* __restrict__ o4, const float* __restrict__ noise1,
+    const float* __restrict__ noise2, StepScalars<float> c, TableF32<false> tab,
+    ZnParams<float> zn, int nx, int ny, int nz, int W, TileGrid g, uint64_t noise_seed,
+    unsigned long long* __restrict__ vac, float esc2)
+{
+    extern __shared__ float lds_crows[];
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
+    unsigned cnt = rk2_complex_rows_tile<TY, NM>(phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1,
+                                                 noise2, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
+                                                 noise_seed, vac != nullptr, esc2, lds_crows);
+    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+}
+
 // Launch geometry shared by the real and complex launchers.
 //   kind TIP_KERNEL_ROWS (real float32 default where it applies): by = y rows per thread
 //   (1, 2 or 4; must divide ny), cx = x planes per block.
@@ -755,7 +991,7 @@ struct RowShape {
     int W, G, TY;
     bool ok;
 };
-RowShape row_shape(int64_t ny, int64_t nz, int ty);
+RowShape row_shape(int64_t ny, int64_t nz, int ty, int max_w = 16);
 int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz);
 StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field, int f64);
 // Resolve a requested geometry (nullable = default) into the tile grid and the block count.

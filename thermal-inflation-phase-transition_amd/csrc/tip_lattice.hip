@@ -185,12 +185,13 @@ extern "C" int tip_lattice_set_geometry(tip_lattice* lat, int kernel, int rows,
                 "tip_lattice_set_geometry: unknown kernel %d", kernel);
     StepGeometry req = {rows, planes_per_block, xcd_remap, kernel};
     if (kernel == TIP_KERNEL_ROWS) {
-        TIP_REQUIRE(lat->kind == TIP_REAL && lat->prec == 4,
-                    "tip_lattice_set_geometry: the row kernel is the real float32 step");
+        TIP_REQUIRE(lat->prec == 4,
+                    "tip_lattice_set_geometry: the row kernel is the float32 step");
         const RowShape rs = row_shape(
-            lat->ny, lat->nz, rows ? rows : default_rows_per_thread(lat->nx, lat->ny, lat->nz));
-        TIP_REQUIRE(rs.ok, "tip_lattice_set_geometry: row kernel needs nz <= 1024 and rows per "
-                           "thread (1, 2, 4) dividing ny");
+            lat->ny, lat->nz, rows ? rows : default_rows_per_thread(lat->nx, lat->ny, lat->nz),
+            lat->kind == TIP_COMPLEX ? 8 : 16);
+        TIP_REQUIRE(rs.ok, "tip_lattice_set_geometry: row kernel needs nz <= 1024 (complex: 512) "
+                           "and rows per thread (1, 2, 4) dividing ny");
         TIP_REQUIRE(planes_per_block >= 0 && planes_per_block <= 4096,
                     "geometry: planes per block out of range");
     } else {

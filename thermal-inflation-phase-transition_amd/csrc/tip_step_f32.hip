@@ -88,13 +88,13 @@ int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz)
     return 1;
 }
 
-RowShape row_shape(int64_t ny, int64_t nz, int ty)
+RowShape row_shape(int64_t ny, int64_t nz, int ty, int max_w)
 {
     RowShape r;
     r.W = (int)((nz + 63) / 64);
     r.G = r.W >= 4 ? 1 : 4 / r.W;
     r.TY = ty;
-    r.ok = r.W <= 16 && (ty == 1 || ty == 2 || ty == 4) && ny % ty == 0;
+    r.ok = r.W <= max_w && (ty == 1 || ty == 2 || ty == 4) && ny % ty == 0;
     return r;
 }
 
@@ -231,6 +231,54 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
     TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && tab.ab,
                 "rk2 c32: NULL field or table");
     TIP_REQUIRE(noise1 != nullptr || noise2 == nullptr, "rk2 c32: noise2 without noise1");
+    const int nm = noise1 ? NOISE_ARRAY : (hash ? NOISE_HASH : NOISE_NONE);
+    const int kind = geom ? geom->kind : TIP_KERNEL_AUTO;
+    TIP_REQUIRE(kind == TIP_KERNEL_AUTO || kind == TIP_KERNEL_ROWS || kind == TIP_KERNEL_FLAT,
+                "rk2 c32: unknown kernel kind %d", kind);
+    if (kind != TIP_KERNEL_FLAT) {
+        // four fields per site: two y rows per thread at most by default (register budget)
+        int ty_req = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
+        if (!(geom && geom->by) && ty_req > 2) ty_req = 2;
+        const RowShape rs = row_shape(ny, nz, ty_req, 8);  // blocks of at most 512 threads
+        TIP_REQUIRE(rs.ok || kind == TIP_KERNEL_AUTO,
+                    "rk2 c32: row kernel needs nz <= 512 and rows per thread (1, 2, 4) dividing "
+                    "ny (got %d rows, ny = %lld)", ty_req, (long long)ny);
+        if (rs.ok) {
+            TileGrid tg;
+            tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
+            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
+            if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
+            TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
+            tg.cx = (int)cx;
+            tg.chunks = (int)((nx + cx - 1) / cx);
+            const int64_t nb64 = (int64_t)tg.tiles * tg.chunks;
+            TIP_REQUIRE(nb64 < (int64_t(1) << 31), "geometry: too many blocks");
+            const int nbr = (int)nb64;
+            tg.xcd_remap = ((geom ? geom->xcd : 1) && nbr % 8 == 0) ? 1 : 0;
+            const size_t lds = (size_t)rs.G * (64 * rs.W) * (8 * rs.TY + 1) * sizeof(float);
+            const dim3 blk(64, rs.W * rs.G);
+            const TableF32<false> t32 = tab.as<false>();
+#define TIP_LAUNCH_CROWS(TYV, NM)                                                              \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM>), dim3(nbr), blk, lds, st, phi1, phi2, pi1, \
+                       pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn, (int)nx, (int)ny,      \
+                       (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_CROWS_NM(TYV)                                      \
+    switch (nm) {                                                     \
+        case NOISE_ARRAY: TIP_LAUNCH_CROWS(TYV, NOISE_ARRAY); break;  \
+        case NOISE_HASH: TIP_LAUNCH_CROWS(TYV, NOISE_HASH); break;    \
+        default: TIP_LAUNCH_CROWS(TYV, NOISE_NONE); break;            \
+    }
+            switch (rs.TY) {
+                case 1: TIP_LAUNCH_CROWS_NM(1); break;
+                case 4: TIP_LAUNCH_CROWS_NM(4); break;
+                default: TIP_LAUNCH_CROWS_NM(2); break;
+            }
+#undef TIP_LAUNCH_CROWS_NM
+#undef TIP_LAUNCH_CROWS
+            TIP_HIP(hipGetLastError());
+            return TIP_OK;
+        }
+    }
     StepGeometry g;
     TileGrid tg;
     int nb;
