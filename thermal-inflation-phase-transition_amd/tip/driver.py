@@ -53,6 +53,7 @@ class RunConfig:
     eta_phys: Optional[float] = None
     no_hubble: bool = True
     cool_to: Optional[float] = None
+    cool_steps: Optional[int] = None  # length of the linear cooling schedule (default Nt)
     potential_type: Optional[str] = None
     nb: float = 20.0
     nf: float = 20.0
@@ -62,6 +63,8 @@ class RunConfig:
     zn_turn_on_T: float = 0.0
     init_rho: float = 0.01
     counterterm: bool = False  # lattice counterterm on the dJ splines (:458-490, :758-795)
+    table_source: str = "spline"  # "spline": the reference's V' (dJ spline, no CW);
+                                  # "exact": float64 series dJ + Coleman-Weinberg (vprime_full)
     precision: str = "f32"
     seed: int = 0
     table_size: int = 16384
@@ -147,7 +150,7 @@ class Run:
             inv_a3 = s["inv_a3"] if not self.cplx else 1.0
             return s["T"], s["eta_eff"], s["inv_a2"], inv_a3
         if cfg.cool_to is not None:
-            T = cfg.T0 + (cfg.cool_to - cfg.T0) * n / max(cfg.Nt, 1)
+            T = cfg.T0 + (cfg.cool_to - cfg.T0) * n / max(cfg.cool_steps or cfg.Nt, 1)
         else:
             T = cfg.T0
         return T, self.eta, 1.0, 1.0
@@ -171,8 +174,13 @@ class Run:
         margin = max(rng * TABLE_MARGIN_FRAC, 20000.0)
         t_lo = max(0.0, lo - margin) if self.cplx else lo - margin
         t_hi = hi + margin
-        table, tmin, dinv = P.build_vprime_table(T, t_lo, t_hi, self.cfg.table_size, self.model,
-                                                 self.spline)
+        if self.cfg.table_source == "exact":
+            grid = np.linspace(t_lo, t_hi, self.cfg.table_size, dtype=np.float64)
+            table = P.vprime_full(grid, T, self.model, include_cw=True)
+            tmin, dinv = float(t_lo), 1.0 / ((t_hi - t_lo) / (self.cfg.table_size - 1))
+        else:
+            table, tmin, dinv = P.build_vprime_table(T, t_lo, t_hi, self.cfg.table_size,
+                                                     self.model, self.spline)
         self.lat.set_table(table, tmin, dinv, odd=False)
         self.table_T, self.table_lo, self.table_hi = T, t_lo, t_hi
         self.rebuilds += 1
@@ -263,7 +271,8 @@ class Run:
             if self.surrogate is not None and self.hubble is None and not (
                     cfg.zn_order > 0 and cfg.zn_turn_on_T > 0 and self.zn_active < 0.5):
                 k = self._chunk(n)
-                dT = 0.0 if cfg.cool_to is None else (cfg.cool_to - cfg.T0) / max(cfg.Nt, 1)
+                dT = (0.0 if cfg.cool_to is None
+                      else (cfg.cool_to - cfg.T0) / max(cfg.cool_steps or cfg.Nt, 1))
                 self.lat.step_mlp(k, self.surrogate, T_start=T, dT_per_step=dT, **kw)
                 T = T + dT * (k - 1)
             elif self.surrogate is not None:

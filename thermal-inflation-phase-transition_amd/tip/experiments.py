@@ -1,7 +1,8 @@
 """Config-level experiments on top of the driver: the ensemble nucleation run (config 3) and
 the surrogate-versus-table comparison under continuous cooling (config 4).
 
-  python -m tip.experiments surrogate-vs-table --n 64 --steps 4000 --seeds 4
+  python -m tip.experiments surrogate-vs-table --n 128 --steps 3000 --seeds 6 --report 100 \
+         --cool-steps 100000 --exact-table
   python -m tip.experiments nucleation --steps 20000
 """
 from __future__ import annotations
@@ -19,21 +20,26 @@ from .driver import Run, RunConfig, nucleation_ensemble
 
 def surrogate_vs_table(surrogate, n: int = 64, steps: int = 4000, T0: float = 7350.0,
                        T1: float = 6770.0, seeds: Sequence[int] = (0, 1, 2, 3),
-                       report: int = 250, esc: float = 1e4, log=None) -> Dict:
-    """Complex field, linear cooling T0 -> T1 over `steps`, an ensemble of seeds run twice:
-    with the reference's table method (host-built 16384-point table on the field range,
-    rebuilt when |dT|/T > 1e-4 or the field leaves the range) and with the inline surrogate
-    at the exact T(t).  Returns per-report-step ensemble means and standard deviations of the
-    true-vacuum fraction and the string count, and whether the two agree within 1 sigma
-    (|mean_s - mean_t| <= sqrt(sd_s^2 + sd_t^2) / sqrt(len(seeds)) ... reported, not imposed
-    here)."""
+                       report: int = 250, esc: float = 1e4, cool_steps: int = None,
+                       methods: Sequence[str] = ("table", "surrogate"), log=None) -> Dict:
+    """Complex field, linear cooling T0 -> T1 over `cool_steps` steps (default `steps`; the
+    config-4 schedule is 10^5 steps, of which the first `steps` are run), an ensemble of seeds
+    run with each method: "table" -- the reference's table method (host-built 16384-point
+    table of the reference's V' on the field range, rebuilt when |dT|/T > 1e-4 or the field
+    leaves the range); "surrogate" -- the inline network at the exact T(t); "table_exact"
+    (optional) -- the table method fed the float64 potential the network was trained on
+    (series J + Coleman-Weinberg), which separates the network's error from the reference's
+    dJ-spline error.  Returns per-report-step ensemble means and standard deviations of the
+    true-vacuum fraction and the string count, and whether table and surrogate agree within
+    the combined 1 sigma, sqrt(sd_t^2 + sd_s^2), at every report step."""
     out = {}
-    for method in ("table", "surrogate"):
+    for method in methods:
         F, S, el, rebuilds = [], [], 0.0, 0
         for sd in seeds:
             cfg = RunConfig(field="complex", Nx=n, Ny=n, Nz=n, Nt=steps, T0=T0, cool_to=T1,
-                            seed=sd, report=report, esc=esc, quiet=True,
+                            cool_steps=cool_steps, seed=sd, report=report, esc=esc, quiet=True,
                             surrogate=surrogate if method == "surrogate" else None,
+                            table_source="exact" if method == "table_exact" else "spline",
                             divergence=0.0)
             run = Run(cfg)
             try:
@@ -78,11 +84,17 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
     a1 = sub.add_parser("surrogate-vs-table")
-    a1.add_argument("--surrogate", default=None, help="trained npz (default: train now)")
+    a1.add_argument("--surrogate", default=None,
+                    help="trained npz (default: the shipped config-4 weights)")
+    a1.add_argument("--train", action="store_true", help="train a fresh network first")
     a1.add_argument("--n", type=int, default=64)
     a1.add_argument("--steps", type=int, default=4000)
     a1.add_argument("--seeds", type=int, default=4)
     a1.add_argument("--report", type=int, default=250)
+    a1.add_argument("--cool-steps", type=int, default=None,
+                    help="length of the cooling schedule (config 4: 100000)")
+    a1.add_argument("--exact-table", action="store_true",
+                    help="also run the table method on the float64 potential")
     a2 = sub.add_parser("nucleation")
     a2.add_argument("--steps", type=int, default=20000)
     a2.add_argument("--report", type=int, default=500)
@@ -91,9 +103,13 @@ def main(argv=None) -> int:
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if a.what == "surrogate-vs-table":
         from .surrogate import Surrogate, train_config4
-        sur = Surrogate.load(a.surrogate) if a.surrogate else train_config4(log=log)[0]
+        sur = (Surrogate.load(a.surrogate) if a.surrogate
+               else train_config4(log=log)[0] if a.train else Surrogate.load())
         res = surrogate_vs_table(sur, n=a.n, steps=a.steps, seeds=range(a.seeds),
-                                 report=a.report, log=log)
+                                 report=a.report, cool_steps=a.cool_steps,
+                                 methods=("table", "surrogate") + (("table_exact",)
+                                                                   if a.exact_table else ()),
+                                 log=log)
     else:
         res = nucleation_ensemble(steps=a.steps, report=a.report, log=log)
     res = _jsonable(res)

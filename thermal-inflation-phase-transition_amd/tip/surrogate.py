@@ -23,6 +23,7 @@ import ctypes as C
 import dataclasses
 import json
 import math
+import os
 from dataclasses import dataclass, field
 from typing import Dict, Optional
 
@@ -226,14 +227,10 @@ class Surrogate:
                  meta=json.dumps(meta))
 
     @classmethod
-    def load(cls, path) -> "Surrogate":
-        d = np.load(path, allow_pickle=False)
-        model = P.ModelParams(**json.loads(str(d["model"])))
-        spec = SurrogateSpec(model=model, T_lo=float(d["T_lo"]), T_hi=float(d["T_hi"]),
-                             T_ref=float(d["T_ref"]), rho_frac=float(d["rho_frac"]),
-                             rho_scale=float(d["rho_scale"]), rho_cut=float(d["rho_cut"]),
-                             include_cw=bool(d["include_cw"]))
-        return cls(spec, params=d["params"])
+    def load(cls, path=None) -> "Surrogate":
+        """A saved network (default: the shipped config-4 weights)."""
+        spec, params = load_params_npz(path or TRAINED_CONFIG4)
+        return cls(spec, params=params)
 
     def error_report(self, n: int = 1 << 16, seed: int = 99) -> Dict[str, float]:
         """Floored relative error of V' against the float64 truth on fresh samples (float32
@@ -251,6 +248,23 @@ class Surrogate:
                 "median_rel_floored": float(np.median(rel)),
                 "max_rel_barrier_region": float(rel_low.max()) if low.any() else 0.0,
                 "n": int(n)}
+
+
+# Weights trained on an MI355X by train_config4(seed=0) (provenance and the error report at
+# training time are stored in the file's "meta" entry).
+TRAINED_CONFIG4 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "surrogate_config4.npz")
+
+
+def load_params_npz(path: str):
+    """(SurrogateSpec, float32 parameters) of a saved network, without touching the GPU."""
+    d = np.load(path, allow_pickle=False)
+    model = P.ModelParams(**json.loads(str(d["model"])))
+    spec = SurrogateSpec(model=model, T_lo=float(d["T_lo"]), T_hi=float(d["T_hi"]),
+                         T_ref=float(d["T_ref"]), rho_frac=float(d["rho_frac"]),
+                         rho_scale=float(d["rho_scale"]), rho_cut=float(d["rho_cut"]),
+                         include_cw=bool(d["include_cw"]))
+    return spec, np.asarray(d["params"], np.float32)
 
 
 def train_config4(epochs: int = 400, n_samples: int = 1 << 20, seed: int = 0, log=None):
