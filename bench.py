@@ -133,12 +133,16 @@ def bench_ensemble(steps: int = 200):
         ens.set_params(i, **prm)
     tune = autotune.tune_ensemble(ens, steps=10, use_cache=False)
     log(f"ensemble autotune best {tune['best']} {tune['us_per_step']:.1f} us/step")
+    sched = autotune.tune_ensemble_schedule(ens, steps=steps, use_cache=False)
+    log(f"ensemble schedule best {sched['best']} bytes/group {sched['us_per_step']:.1f} us/step "
+        f"({sched['table']})")
     rng = np.random.default_rng(0)
     for i, s in enumerate(shapes):
         ens.upload(i, "phi", 0.01 * rng.normal(size=s))
         ens.upload(i, "pi", np.zeros(s))
     sites = float(sum(int(np.prod(s)) for s in shapes))
-    out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"]}
+    out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"],
+           "group_bytes": sched["best"]}
     for mode in ("graph", "batched", "sequential"):
         ens.step(10, mode=mode, timed=True)
         t0 = time.perf_counter()

@@ -279,6 +279,12 @@ int tip_ensemble_set_params(tip_ensemble *ens, int inst, const tip_step_params *
  * each walking a contiguous item range (0 = one block per item). */
 int tip_ensemble_set_geometry(tip_ensemble *ens, int kernel, int rows, int planes_per_block,
                               int xcd_remap, int persistent_blocks);
+/* Step schedule of the batched modes: with group_bytes > 0 the instances are cut, in order,
+ * into groups holding at most group_bytes of field data (phi and pi, both buffers: 16 B per
+ * site) and each group runs all nsteps of a call before the next group starts, so a group's
+ * fields stay in the GPU's last-level cache (Infinity Cache) from one step to the next.
+ * 0 (default) = one launch per step over every instance.  Results are identical either way. */
+int tip_ensemble_set_schedule(tip_ensemble *ens, int64_t group_bytes);
 int tip_ensemble_step(tip_ensemble *ens, int64_t nsteps, int mode, float *elapsed_ms);
 int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long *host_counts);
 int tip_ensemble_step_count(tip_ensemble *ens, uint64_t *step);

@@ -85,6 +85,24 @@ def test_geometries_identical(geometry):
     ens_b.close()
 
 
+@pytest.mark.parametrize("group_bytes", [1, 40000, 1 << 30])
+def test_grouped_schedule_identical(group_bytes):
+    """Instances stepped group by group (each group through all steps of a call) give the same
+    fields as one launch per step over all instances."""
+    ens_a, _, _, _ = _setup()
+    ens_a.step(5, mode="batched")
+    ref = _state(ens_a)
+    for mode in ("batched", "graph"):
+        ens_b, _, _, _ = _setup()
+        ens_b.set_schedule(group_bytes)
+        ens_b.step(2, mode=mode)
+        ens_b.step(3, mode=mode)
+        assert _same(_state(ens_b), ref), (group_bytes, mode)
+        assert ens_b.step_count == 5
+        ens_b.close()
+    ens_a.close()
+
+
 def test_graph_replays_each_checked():
     ens_e, _, _, _ = _setup()
     ens_g, _, _, _ = _setup()

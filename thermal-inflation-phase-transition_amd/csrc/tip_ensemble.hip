@@ -134,6 +134,10 @@ struct tip_ensemble {
     // per thread, 0 = default), cx = x planes per work item
     int kind = TIP_KERNEL_AUTO, by = 0, cx = 8, xcd = 1, persistent = 0;
     int use_rows = 0, ty = 1, flat_by = 4;  // resolved by build_items
+    // schedule: instances are stepped in groups of at most group_bytes of field data (both
+    // buffers of phi and pi); each group runs all steps of a call before the next group
+    int64_t group_bytes = 0;
+    std::vector<std::pair<int, int>> groups;  // item ranges [first, last)
     EnsItem* dev_items = nullptr;
     int nitems = 0;
     bool items_dirty = true;
@@ -231,8 +235,18 @@ int build_items(tip_ensemble* E)
     int rc = resolve_kernel(E);
     if (rc) return rc;
     std::vector<EnsItem> items;
+    E->groups.clear();
+    int64_t gbytes = 0;
+    size_t gfirst = 0;
     for (int i = 0; i < E->n; ++i) {
         const int nx = (int)E->dims[3 * i], ny = (int)E->dims[3 * i + 1], nz = (int)E->dims[3 * i + 2];
+        const int64_t ibytes = (int64_t)inst_sites(E, i) * 16;
+        if (E->group_bytes > 0 && gbytes > 0 && gbytes + ibytes > E->group_bytes) {
+            E->groups.push_back({(int)gfirst, (int)items.size()});
+            gfirst = items.size();
+            gbytes = 0;
+        }
+        gbytes += ibytes;
         int tiles;
         if (E->use_rows) {
             int W = (nz + 63) / 64;
@@ -246,6 +260,7 @@ int build_items(tip_ensemble* E)
         for (int x0 = 0; x0 < nx; x0 += E->cx)
             for (int t = 0; t < tiles; ++t) items.push_back({i, t, x0, 0});
     }
+    E->groups.push_back({(int)gfirst, (int)items.size()});
     E->desc_dirty = true;
     rc = sync_descriptors(E);
     if (rc) return rc;
@@ -262,16 +277,18 @@ int build_items(tip_ensemble* E)
     return TIP_OK;
 }
 
-int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long* vac, float esc)
+int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long* vac, float esc,
+                   int item0, int nit)
 {
-    int nb = E->nitems;
+    const EnsItem* items = E->dev_items + item0;
+    int nb = nit;
     if (E->persistent > 0 && E->persistent < nb) nb = E->persistent;
     const int remap = (E->xcd && nb % 8 == 0) ? 1 : 0;
     if (E->use_rows) {
         const size_t lds = (size_t)256 * (4 * E->ty + 1) * sizeof(float);
 #define TIP_LAUNCH_ENSR(TYV, ODD)                                                                \
     hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD>), dim3(nb), dim3(64, 4), lds, E->stream, \
-                       E->dev_inst, E->dev_items, E->nitems, E->cx, parity, E->dev_step,        \
+                       E->dev_inst, items, nit, E->cx, parity, E->dev_step,                     \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENSR_TY(ODD)                        \
     switch (E->ty) {                                   \
@@ -291,7 +308,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
     }
 #define TIP_LAUNCH_ENS(BYV, ODD)                                                                \
     hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, E->stream,     \
-                       E->dev_inst, E->dev_items, E->nitems, E->cx, parity, E->dev_step,       \
+                       E->dev_inst, items, nit, E->cx, parity, E->dev_step,                    \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENS_BY(ODD)                         \
     switch (E->flat_by) {                              \
@@ -308,6 +325,25 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
 #undef TIP_LAUNCH_ENS_BY
 #undef TIP_LAUNCH_ENS
     TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
+
+// All launches of nsteps steps starting at buffer parity par0 (no counter bump).
+int issue_steps(tip_ensemble* E, int64_t nsteps, int par0)
+{
+    if (E->groups.size() > 1) {
+        for (const auto& g : E->groups)
+            for (int64_t s = 0; s < nsteps; ++s) {
+                int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, nullptr, 0.0f, g.first,
+                                        g.second - g.first);
+                if (rc) return rc;
+            }
+        return TIP_OK;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) {
+        int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, nullptr, 0.0f, 0, E->nitems);
+        if (rc) return rc;
+    }
     return TIP_OK;
 }
 
@@ -475,6 +511,15 @@ extern "C" int tip_ensemble_set_geometry(tip_ensemble* E, int kernel, int rows,
     return TIP_OK;
 }
 
+extern "C" int tip_ensemble_set_schedule(tip_ensemble* E, int64_t group_bytes)
+{
+    TIP_REQUIRE(E, "tip_ensemble_set_schedule: NULL");
+    TIP_REQUIRE(group_bytes >= 0, "tip_ensemble_set_schedule: group_bytes < 0");
+    E->group_bytes = group_bytes;
+    E->items_dirty = true;
+    return TIP_OK;
+}
+
 extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, float* elapsed_ms)
 {
     TIP_REQUIRE(E, "tip_ensemble_step: NULL");
@@ -508,11 +553,9 @@ extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, floa
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
         TIP_HIP(hipGetLastError());
     } else if (mode == TIP_ENS_BATCHED || nsteps == 0) {
-        for (int64_t s = 0; s < nsteps; ++s) {
-            rc = launch_batched(E, E->parity, (int)s, nullptr, 0.0f);
-            if (rc) return rc;
-            E->parity ^= 1;
-        }
+        rc = issue_steps(E, nsteps, E->parity);
+        if (rc) return rc;
+        if (nsteps & 1) E->parity ^= 1;
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
         TIP_HIP(hipGetLastError());
     } else {
@@ -522,14 +565,10 @@ extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, floa
         if (itg == E->graphs.end()) {
             hipGraph_t g;
             TIP_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-            int par = E->parity;
-            for (int64_t s = 0; s < nsteps; ++s) {
-                rc = launch_batched(E, par, (int)s, nullptr, 0.0f);
-                if (rc) {
-                    (void)hipStreamEndCapture(E->stream, &g);
-                    return rc;
-                }
-                par ^= 1;
+            rc = issue_steps(E, nsteps, E->parity);
+            if (rc) {
+                (void)hipStreamEndCapture(E->stream, &g);
+                return rc;
             }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step,
                                (int)nsteps);

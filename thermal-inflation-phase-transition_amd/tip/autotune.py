@@ -107,3 +107,29 @@ def tune_ensemble(ens, candidates: Optional[Iterable] = None, steps: int = 10,
            "table": [[list(c), t * 1e3] for t, c in results]}
     _store_cache(key, rec)
     return rec
+
+
+def tune_ensemble_schedule(ens, candidates: Iterable[int] = (0, 96 << 20, 160 << 20, 224 << 20,
+                                                             320 << 20),
+                           steps: int = 50, mode: str = "graph", use_cache: bool = True) -> dict:
+    """Time each group size of the ensemble's step schedule (bytes of field data per group,
+    0 = all instances in every launch) over `steps`-step calls and apply the fastest."""
+    key = f"ensemble_schedule:{sorted(set(ens.shapes))}:{len(ens.shapes)}:{mode}:{steps}"
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            ens.set_schedule(hit["best"])
+            return hit
+    results = []
+    for gb in candidates:
+        ens.set_schedule(int(gb))
+        ens.step(steps, mode=mode, timed=True)  # capture / warm
+        ms = ens.step(steps, mode=mode, timed=True)
+        results.append((ms / steps, int(gb)))
+    results.sort()
+    best = results[0][1]
+    ens.set_schedule(best)
+    rec = {"best": best, "us_per_step": results[0][0] * 1e3,
+           "table": [[c, t * 1e3] for t, c in results]}
+    _store_cache(key, rec)
+    return rec

@@ -57,6 +57,11 @@ class Ensemble:
         _lib.call("tip_ensemble_set_geometry", self._h, int(kernel), int(rows),
                   int(planes_per_block), int(bool(xcd_remap)), int(persistent_blocks))
 
+    def set_schedule(self, group_bytes: int = 0) -> None:
+        """Step instances in groups of at most group_bytes of field data (16 B per site), each
+        group through all steps of a call before the next (0 = all together)."""
+        _lib.call("tip_ensemble_set_schedule", self._h, int(group_bytes))
+
     def step(self, nsteps: int, mode: str = "graph", timed: bool = False):
         ms = C.c_float(0.0)
         _lib.call("tip_ensemble_step", self._h, int(nsteps), MODES[mode],
