@@ -114,7 +114,8 @@ def test_f64_100_steps_c0_within_1e12():
     assert O.vacuum_count(g, esc) == O.vacuum_count(a, esc)
 
 
-@pytest.mark.parametrize("shape", [(5, 7, 9), (3, 70, 130), (1, 1, 1), (2, 3, 64), (33, 5, 65)])
+@pytest.mark.parametrize("shape", [(5, 7, 9), (3, 70, 130), (1, 1, 1), (2, 3, 64), (33, 5, 65),
+                                   (2, 3, 1100)])
 def test_f64_ragged_shapes_one_step_explicit_noise(shape):
     """Explicit noise array (the reference's calling form) on ragged shapes: 1e-14."""
     tip, K, _, DA, _ = _tip()
@@ -131,6 +132,19 @@ def test_f64_ragged_shapes_one_step_explicit_noise(shape):
     K.rk2_step_table(g, h, 0.25, 2.0, 7.0, 1000.0, noise, np.empty_like(g), np.empty_like(h), 0.9,
                      t, tmin, dinv, len(t))
     assert rel(g, a) < 1e-14 and rel(h, b) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(3, 70, 130), (4, 6, 1100), (5, 7, 9)])
+def test_f64_hash_noise_ragged_shapes(shape):
+    """Inline hash noise on ragged shapes: 3 steps within 1e-12 of the oracle."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    phi0 = rng.normal(size=shape) * 3e3
+    pi0 = rng.normal(size=shape) * 1e3
+    t, tmin, dinv = c0_table(1024)
+    prm = c0_params()
+    a, b = O.run_real(phi0, pi0, 3, prm, t, tmin, dinv, odd=True, seed=9)
+    g, h = _run_gpu_real(phi0, pi0, 3, prm, t, tmin, dinv, True, np.float64, seed=9)
+    assert rel(g, a) <= 1e-12 and rel(h, b) <= 1e-12
 
 
 def test_f32_error_against_f64():
