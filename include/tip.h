@@ -334,12 +334,17 @@ int tip_rk2_fused_c32_mlp(const float *phi1, const float *phi2, const float *pi1
                           unsigned long long *vac_count, double vac_esc, void *stream);
 
 /* nsteps complex-field steps of a device-resident lattice with V'(rho, T) from the surrogate,
- * T = T_start + dT_per_step * n at step n (continuous cooling, no re-tabulation).  The noise
- * amplitude p->noise_scale is the one at T_start; step n uses noise_scale * sqrt(T_n / T_start),
- * the T dependence of sqrt(2 eta T dt / (mu^2 dx^3)).  T must stay positive. */
+ * T = T_start + dT_per_step * n at step n (continuous cooling, no re-tabulation from the
+ * potential).  The noise amplitude p->noise_scale is the one at T_start; step n uses
+ * noise_scale * sqrt(T_n / T_start), the T dependence of sqrt(2 eta T dt / (mu^2 dx^3)).
+ * T must stay positive.  grid_points = 0: the network is evaluated inline at every site and
+ * stage; grid_points >= 2: each step first evaluates the network at T_n on that many points of
+ * its compressed rho coordinate (a device buffer the lattice keeps) and the step interpolates
+ * them (same network and T(t); about 10x faster; nz <= 512). */
 int tip_lattice_step_mlp(tip_lattice *lat, const tip_step_params *p, const float *mlp_params,
                          const tip_mlp_features *fm, double T_start, double dT_per_step,
-                         int64_t nsteps, uint64_t step0, uint64_t seed, float *elapsed_ms);
+                         int64_t nsteps, uint64_t step0, uint64_t seed, int64_t grid_points,
+                         float *elapsed_ms);
 
 #if defined(TIP_BUILDING) && defined(__GNUC__)
 #pragma GCC visibility pop

@@ -156,3 +156,29 @@ def test_gain_over_table_and_fixed_temperature(trained):
     err_net = np.max(np.abs(netv - truth) / (r * 1e6))
     print(f"held-table error {err_held:.3e}, network error {err_net:.3e}")
     assert err_net < 0.2 * err_held
+
+
+def test_grid_mode_matches_inline():
+    """The per-step grid of the network (16385 points of the compressed coordinate, rebuilt at
+    every step's T) gives the inline evaluation's fields: 10 cooling steps from a state that
+    spans the barrier and the vacuum, within 1e-5 relative."""
+    from tip import surrogate as S
+    from tip.lattice import Lattice
+    sur = S.Surrogate.load()
+    rng = np.random.default_rng(31)
+    shape = (16, 24, 64)
+    rho = np.exp(rng.uniform(np.log(1.0), np.log(1.2e11), size=shape))
+    th = rng.uniform(0, 2 * np.pi, size=shape)
+    init = {"phi1": rho * np.cos(th), "phi2": rho * np.sin(th),
+            "pi1": rng.normal(size=shape) * 1e2, "pi2": rng.normal(size=shape) * 1e2}
+    kw = dict(dt=0.25, dx=1.0, eta_eff=1.0, mu=1000.0, noise_scale=300.0, seed=2)
+    outs = []
+    for grid in (0, 16385):
+        lat = Lattice(shape, np.float32, complex_field=True)
+        for k, v in init.items():
+            lat.upload(k, v)
+        lat.step_mlp(10, sur, T_start=7350.0, dT_per_step=-0.5, grid_points=grid, **kw)
+        outs.append([lat.download(k).astype(np.float64) for k in ("phi1", "phi2", "pi1", "pi2")])
+        lat.close()
+    for a, b in zip(*outs):
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)

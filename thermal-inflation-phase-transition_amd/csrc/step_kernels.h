@@ -759,14 +759,50 @@ __device__ __forceinline__ void gauss_pair_f32_l2(uint64_t seed, uint32_t word_m
     zs = r * __builtin_amdgcn_sinf(u2);
 }
 
-__device__ __forceinline__ void complex_force_f32(__amdgpu_buffer_rsrc_t T,
-                                                  const TableF32<false>& tab,
-                                                  const ZnParams<float>& zn, float a, float b,
-                                                  float& d1, float& d2)
+// V'(rho) of the complex field's radial potential: the float32 segment table ...
+struct TableRadial {
+    TableF32<false> t;
+    __device__ __forceinline__ float operator()(float rho) const
+    {
+        const __amdgpu_buffer_rsrc_t T = make_rsrc(t.ab, (uint32_t)(t.last + 1) * 8u);
+        return seg_eval<false>(seg_load<false>(T, t, rho), rho);
+    }
+};
+
+// ... or the learned network at this step's temperature, read from its per-step grid in the
+// compressed coordinate g = asinh(rho / rho_s) * gscale (segments (y_i, y_{i+1} - y_i), linear
+// in g), with the exact tree term and the fixed envelope of the learned part:
+// V' = rho (lam rho^2 - m2 + out_scale y / (1 + (rho / rho_cut)^2)), formed in float64.
+struct MlpGridRadial {
+    const float2* seg;
+    int last;  // number of segments - 1
+    float inv_rho_s, gscale, out_scale, inv_rho_cut;
+    double lam, m2;
+    __device__ __forceinline__ float operator()(float rho) const
+    {
+        const __amdgpu_buffer_rsrc_t Y = make_rsrc(seg, (uint32_t)(last + 1) * 8u);
+        const float s = rho * inv_rho_s;
+        const float g = __logf(s + __builtin_sqrtf(fmaf(s, s, 1.0f))) * gscale;
+        const int idx = (int)__builtin_amdgcn_fmed3f(g, 0.0f, (float)last);
+        using f2 = float __attribute__((ext_vector_type(2)));
+        const f2 ab =
+            __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(Y, idx * 8, 0, 0));
+        const float y = fmaf(ab.y, g - (float)idx, ab.x);
+        const float q = rho * inv_rho_cut;
+        const float th = out_scale * y / fmaf(q, q, 1.0f);
+        const double r = (double)rho;
+        return (float)(r * (fma(lam * r, r, -m2) + (double)th));
+    }
+};
+
+// dV_i = V'(rho) phi_i / max(rho, 1e-30) plus the Z_N term.
+template <typename Radial>
+__device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnParams<float>& zn,
+                                                  float a, float b, float& d1, float& d2)
 {
     const float rho = __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
     const float inv = __builtin_amdgcn_rcpf(fmaxf(rho, 1e-30f));
-    const float w = seg_eval<false>(seg_load<false>(T, tab, rho), rho) * inv;
+    const float w = rad(rho) * inv;
     d1 = w * a;
     d2 = w * b;
     if (zn.active && rho > 1e-20f) {
@@ -789,12 +825,12 @@ __device__ __forceinline__ void complex_force_f32(__amdgpu_buffer_rsrc_t T,
     }
 }
 
-template <int TY, int NM>
+template <int TY, int NM, typename Radial>
 __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
     float* __restrict__ o3, float* __restrict__ o4, const float* __restrict__ noise1,
-    const float* __restrict__ noise2, const StepScalars<float>& c, const TableF32<false>& tab,
+    const float* __restrict__ noise2, const StepScalars<float>& c, const Radial& rad,
     const ZnParams<float>& zn, int nx, int ny, int nz, int W, int tile, int x0, int x1,
     uint64_t noise_seed, bool count, float esc2, float* __restrict__ lds)
 {
@@ -822,7 +858,6 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     char* O[NF] = {reinterpret_cast<char*>(o1), reinterpret_cast<char*>(o2),
                    reinterpret_cast<char*>(o3), reinterpret_cast<char*>(o4)};
     auto at = [&](const char* bp, int x) { return make_rsrc(bp + (size_t)x * plane, pb); };
-    const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
     uint32_t off[TY], offs[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
@@ -900,14 +935,14 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 gauss_pair_f32_l2((col0K + (uint64_t)r * rowK + xK) ^ noise_seed, c.word_mask, g1,
                                   g2);
             float d1, d2;
-            complex_force_f32(T, tab, zn, a, bb, d1, d2);
+            complex_force_f32(rad, zn, a, bb, d1, d2);
             const float k1a = fmaf(-c.inv_mu2, d1, fmaf(c.c_lap, t1a, -c.eta * u));
             const float k1b = fmaf(-c.inv_mu2, d2, fmaf(c.c_lap, t1b, -c.eta * v));
             const float ga = fmaf(c.half_dt, k1a, u), gb = fmaf(c.half_dt, k1b, v);
             const float t2a = fmaf(c.half_dt, fmaf(-6.0f, u, sum[2]), t1a);
             const float t2b = fmaf(c.half_dt, fmaf(-6.0f, v, sum[3]), t1b);
             float e1, e2;
-            complex_force_f32(T, tab, zn, fa, fb, e1, e2);
+            complex_force_f32(rad, zn, fa, fb, e1, e2);
             const float k2a = fmaf(-c.inv_mu2, e1, fmaf(c.c_lap, t2a, -c.eta * ga));
             const float k2b = fmaf(-c.inv_mu2, e2, fmaf(c.c_lap, t2b, -c.eta * gb));
             const float na = fmaf(c.dt, ga, a), nb = fmaf(c.dt, gb, bb);
@@ -955,13 +990,13 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     return cnt;
 }
 
-template <int TY, int NM>
+template <int TY, int NM, typename Radial>
 __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
     float* __restrict__ o3, float* __restrict__ o4, const float* __restrict__ noise1,
-    const float* __restrict__ noise2, StepScalars<float> c, TableF32<false> tab,
-    ZnParams<float> zn, int nx, int ny, int nz, int W, TileGrid g, uint64_t noise_seed,
+    const float* __restrict__ noise2, StepScalars<float> c, Radial rad, ZnParams<float> zn,
+    int nx, int ny, int nz, int W, TileGrid g, uint64_t noise_seed,
     unsigned long long* __restrict__ vac, float esc2)
 {
     extern __shared__ float lds_crows[];
@@ -969,9 +1004,9 @@ __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_complex_rows_tile<TY, NM>(phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1,
-                                                 noise2, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
-                                                 noise_seed, vac != nullptr, esc2, lds_crows);
+    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial>(
+        phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, rad, zn, nx, ny, nz, W, tp, x0,
+        x1, noise_seed, vac != nullptr, esc2, lds_crows);
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
@@ -1018,6 +1053,16 @@ int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, c
                        const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
                        uint64_t noise_seed, int hash, unsigned long long* vac, float esc2,
                        hipStream_t st);
+// The same step with V'(rho, T) from a per-step grid of the network (tip_mlp.hip): one small
+// kernel tabulates the network at T on npts points of its compressed coordinate into `grid`
+// ((npts - 1) float2), then the row-tile step reads it.
+int launch_complex_mlp_grid(const float* phi1, const float* phi2, const float* pi1,
+                            const float* pi2, float* o1, float* o2, float* o3, float* o4,
+                            const StepScalars<float>& c, const float* mlp_params,
+                            const tip_mlp_features* fm, double T, const ZnParams<float>& zn,
+                            int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
+                            unsigned long long* vac, float esc2, float2* grid, int64_t npts,
+                            hipStream_t st);
 ZnParams<float> make_zn32(double zn_n, double zn_dv, double zn_active);
 
 }  // namespace tip

@@ -21,6 +21,8 @@ struct tip_lattice {
     unsigned long long* counter = nullptr;  // 2 entries
     double* mm = nullptr;                   // 2 entries
     tip::StepGeometry geom = {0, 0, 1, TIP_KERNEL_AUTO};  // defaults for the lattice size
+    float2* mlp_grid = nullptr;  // per-step grid of the surrogate ((npts - 1) float2)
+    int64_t mlp_grid_cap = 0;
 };
 
 using namespace tip;
@@ -40,6 +42,7 @@ int release(tip_lattice* L)
     if (L->table_ab) (void)hipFree(L->table_ab);
     if (L->counter) (void)hipFree(L->counter);
     if (L->mm) (void)hipFree(L->mm);
+    if (L->mlp_grid) (void)hipFree(L->mlp_grid);
     if (L->ev0) (void)hipEventDestroy(L->ev0);
     if (L->ev1) (void)hipEventDestroy(L->ev1);
     if (L->stream) (void)hipStreamDestroy(L->stream);
@@ -258,11 +261,21 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
 extern "C" int tip_lattice_step_mlp(tip_lattice* L, const tip_step_params* p,
                                     const float* mlp_params, const tip_mlp_features* fm, double T,
                                     double dT_per_step, int64_t nsteps, uint64_t step0,
-                                    uint64_t seed, float* elapsed_ms)
+                                    uint64_t seed, int64_t grid_points, float* elapsed_ms)
 {
     TIP_REQUIRE(L && p && mlp_params && fm, "tip_lattice_step_mlp: NULL");
     TIP_REQUIRE(L->kind == TIP_COMPLEX, "tip_lattice_step_mlp: needs a complex lattice");
     TIP_REQUIRE(nsteps >= 0, "tip_lattice_step_mlp: nsteps < 0");
+    TIP_REQUIRE(grid_points == 0 || (grid_points >= 2 && grid_points <= (int64_t(1) << 24)),
+                "tip_lattice_step_mlp: grid_points must be 0 or in [2, 2^24]");
+    if (grid_points > 0 && grid_points - 1 > L->mlp_grid_cap) {
+        TIP_HIP(hipStreamSynchronize(L->stream));
+        if (L->mlp_grid) TIP_HIP(hipFree(L->mlp_grid));
+        L->mlp_grid = nullptr;
+        L->mlp_grid_cap = 0;
+        TIP_HIP(hipMalloc(&L->mlp_grid, (size_t)(grid_points - 1) * sizeof(float2)));
+        L->mlp_grid_cap = grid_points - 1;
+    }
     TIP_REQUIRE(T > 0.0 && T + dT_per_step * (double)nsteps > 0.0,
                 "tip_lattice_step_mlp: temperature must stay positive");
     const ZnParams<float> zn32 = make_zn32(L->zn_n, L->zn_dv, L->zn_active);
@@ -275,11 +288,18 @@ extern "C" int tip_lattice_step_mlp(tip_lattice* L, const tip_step_params* p,
         const double Tn = T + dT_per_step * (double)n;
         const StepScalars<float> c32 = make_scalars<float>(
             p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale * sqrt(Tn / T));
-        int rc = launch_complex_mlp(
-            (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3],
-            (float*)out[0], (float*)out[1], (float*)out[2], (float*)out[3], c32, mlp_params, fm,
-            Tn, zn32, L->nx, L->ny, L->nz,
-            tip_noise_seed(step0 + (uint64_t)n, seed), hash, nullptr, 0.0f, L->stream);
+        int rc = grid_points > 0
+            ? launch_complex_mlp_grid(
+                  (const float*)in[0], (const float*)in[1], (const float*)in[2],
+                  (const float*)in[3], (float*)out[0], (float*)out[1], (float*)out[2],
+                  (float*)out[3], c32, mlp_params, fm, Tn, zn32, L->nx, L->ny, L->nz,
+                  tip_noise_seed(step0 + (uint64_t)n, seed), hash, nullptr, 0.0f, L->mlp_grid,
+                  grid_points, L->stream)
+            : launch_complex_mlp(
+                  (const float*)in[0], (const float*)in[1], (const float*)in[2],
+                  (const float*)in[3], (float*)out[0], (float*)out[1], (float*)out[2],
+                  (float*)out[3], c32, mlp_params, fm, Tn, zn32, L->nx, L->ny, L->nz,
+                  tip_noise_seed(step0 + (uint64_t)n, seed), hash, nullptr, 0.0f, L->stream);
         if (rc) return rc;
         L->cur ^= 1;
     }

@@ -330,7 +330,88 @@ extern "C" int tip_mlp_tabulate(const float* params, const tip_mlp_features* fm,
     return TIP_OK;
 }
 
+namespace {
+
+// Grid of the network at one temperature: segment i of y(g) on g in [i, i+1], g the grid
+// coordinate of f1 = i * f1_step.
+__global__ void k_mlp_grid(const float* __restrict__ P, float f2, int nseg, float f1_step,
+                           float2* __restrict__ seg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    const float y0 = mlp_eval(P, (float)i * f1_step, f2);
+    const float y1 = mlp_eval(P, (float)(i + 1) * f1_step, f2);
+    seg[i] = make_float2(y0, y1 - y0);
+}
+
+}  // namespace
+
 namespace tip {
+
+// Largest feature value of the per-step grid: rho up to rho_max^1.05 in the compressed
+// coordinate; beyond it the last segment extrapolates (the learned part is negligible there,
+// its envelope is (rho_cut / rho)^2 < 1e-12).
+constexpr float kGridF1Max = 1.05f;
+
+int launch_complex_mlp_grid(const float* phi1, const float* phi2, const float* pi1,
+                            const float* pi2, float* o1, float* o2, float* o3, float* o4,
+                            const StepScalars<float>& c, const float* mlp_params,
+                            const tip_mlp_features* fm, double T, const ZnParams<float>& zn,
+                            int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed, int hash,
+                            unsigned long long* vac, float esc2, float2* grid, int64_t npts,
+                            hipStream_t st)
+{
+    int rc = check_fm(fm);
+    if (rc) return rc;
+    rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(phi1 && phi2 && pi1 && pi2 && o1 && o2 && o3 && o4 && mlp_params && grid,
+                "rk2 c32 mlp grid: NULL pointer");
+    TIP_REQUIRE(npts >= 2 && npts <= (int64_t(1) << 24), "rk2 c32 mlp grid: npts out of range");
+    const int ty = default_rows_per_thread(nx, ny, nz) > 1 && ny % 2 == 0 ? 2 : 1;
+    const RowShape rs = row_shape(ny, nz, ty, 8);
+    TIP_REQUIRE(rs.ok, "rk2 c32 mlp grid: needs nz <= 512");
+    const FeatureMap f = make_fm(fm);
+    const int nseg = (int)(npts - 1);
+    const float f1_step = kGridF1Max / (float)nseg;
+    hipLaunchKernelGGL(k_mlp_grid, dim3((nseg + 255) / 256), dim3(256), 0, st, mlp_params,
+                       (float)((T - fm->T_mid) / fm->T_half), nseg, f1_step, grid);
+    TIP_HIP(hipGetLastError());
+    MlpGridRadial rad;
+    rad.seg = grid;
+    rad.last = nseg - 1;
+    rad.inv_rho_s = f.inv_rho_s;
+    rad.gscale = (float)((double)f.inv_norm * (double)nseg / (double)kGridF1Max);
+    rad.out_scale = f.out_scale;
+    rad.inv_rho_cut = f.inv_rho_cut;
+    rad.lam = f.lam;
+    rad.m2 = f.m2;
+    TileGrid tg;
+    tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
+    int64_t cx = (nx * tg.tiles) / 1024;
+    cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
+    tg.cx = (int)cx;
+    tg.chunks = (int)((nx + cx - 1) / cx);
+    const int64_t nb64 = (int64_t)tg.tiles * tg.chunks;
+    TIP_REQUIRE(nb64 < (int64_t(1) << 31), "rk2 c32 mlp grid: too many blocks");
+    tg.xcd_remap = nb64 % 8 == 0 ? 1 : 0;
+    const size_t lds = (size_t)rs.G * (64 * rs.W) * (8 * rs.TY + 1) * sizeof(float);
+    const dim3 blk(64, rs.W * rs.G);
+#define TIP_LAUNCH_MG(TYV, NM)                                                                   \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, MlpGridRadial>), dim3((int)nb64), blk, lds, \
+                       st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, rad, zn,  \
+                       (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+    if (rs.TY == 2) {
+        if (hash) TIP_LAUNCH_MG(2, NOISE_HASH);
+        else TIP_LAUNCH_MG(2, NOISE_NONE);
+    } else {
+        if (hash) TIP_LAUNCH_MG(1, NOISE_HASH);
+        else TIP_LAUNCH_MG(1, NOISE_NONE);
+    }
+#undef TIP_LAUNCH_MG
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
+}
 
 int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, const float* pi2,
                        float* o1, float* o2, float* o3, float* o4, const StepScalars<float>& c,

@@ -257,11 +257,12 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             tg.xcd_remap = ((geom ? geom->xcd : 1) && nbr % 8 == 0) ? 1 : 0;
             const size_t lds = (size_t)rs.G * (64 * rs.W) * (8 * rs.TY + 1) * sizeof(float);
             const dim3 blk(64, rs.W * rs.G);
-            const TableF32<false> t32 = tab.as<false>();
-#define TIP_LAUNCH_CROWS(TYV, NM)                                                              \
-    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM>), dim3(nbr), blk, lds, st, phi1, phi2, pi1, \
-                       pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn, (int)nx, (int)ny,      \
-                       (int)nz, rs.W, tg, noise_seed, vac, esc2)
+            TableRadial t32;
+            t32.t = tab.as<false>();
+#define TIP_LAUNCH_CROWS(TYV, NM)                                                                \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial>), dim3(nbr), blk, lds, st, phi1, \
+                       phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn, (int)nx,      \
+                       (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
 #define TIP_LAUNCH_CROWS_NM(TYV)                                      \
     switch (nm) {                                                     \
         case NOISE_ARRAY: TIP_LAUNCH_CROWS(TYV, NOISE_ARRAY); break;  \

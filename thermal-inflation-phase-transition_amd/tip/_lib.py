@@ -122,7 +122,7 @@ SIGNATURES = {
     "tip_mlp_vprime": (_int, [_vp, C.POINTER(MlpFeatures), _vp, _vp, _vp, _i64, _vp]),
     "tip_mlp_tabulate": (_int, [_vp, C.POINTER(MlpFeatures), _d, _d, _d, _i64, _vp, _vp]),
     "tip_lattice_step_mlp": (_int, [_vp, C.POINTER(StepParams), _vp, C.POINTER(MlpFeatures), _d,
-                                     _d, _i64, _u64, _u64, C.POINTER(C.c_float)]),
+                                     _d, _i64, _u64, _u64, _i64, C.POINTER(C.c_float)]),
     "tip_rk2_fused_c32_mlp": (_int, [
         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp,
         C.POINTER(MlpFeatures), _d, _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp, _d, _vp]),

@@ -73,6 +73,7 @@ class RunConfig:
     steps: int = 0           # snapshot interval (0 = none)
     outdir: Optional[str] = None
     surrogate: Optional[str] = None
+    surrogate_grid: int = 0  # 0: network inline per site; > 0: per-step grid of that many points
     divergence: float = 1e6  # |phi|max threshold of the reference's guard (0 disables)
     guard_every: int = 100000  # latticeSimeRescale_numba.py:3883
     quiet: bool = False
@@ -273,11 +274,13 @@ class Run:
                 k = self._chunk(n)
                 dT = (0.0 if cfg.cool_to is None
                       else (cfg.cool_to - cfg.T0) / max(cfg.cool_steps or cfg.Nt, 1))
-                self.lat.step_mlp(k, self.surrogate, T_start=T, dT_per_step=dT, **kw)
+                self.lat.step_mlp(k, self.surrogate, T_start=T, dT_per_step=dT,
+                                  grid_points=cfg.surrogate_grid, **kw)
                 T = T + dT * (k - 1)
             elif self.surrogate is not None:
                 k = 1
-                self.lat.step_mlp(1, self.surrogate, T_start=T, **kw)
+                self.lat.step_mlp(1, self.surrogate, T_start=T, grid_points=cfg.surrogate_grid,
+                                  **kw)
             else:
                 k = 1
                 self.maybe_rebuild(T)

@@ -93,9 +93,11 @@ class Lattice:
     def step_mlp(self, nsteps: int, surrogate, *, T_start: float, dT_per_step: float = 0.0,
                  dt: float, dx: float, eta_eff: float, mu: float, inv_a2: float = 1.0,
                  noise_scale: float = 0.0, seed: int = 0, step0: Optional[int] = None,
-                 timed: bool = False) -> Optional[float]:
+                 grid_points: int = 0, timed: bool = False) -> Optional[float]:
         """Complex field: advance nsteps with V' from a trained tip.surrogate.Surrogate at
-        T = T_start + dT_per_step * n (continuous in T, no table)."""
+        T = T_start + dT_per_step * n (continuous in T, no table of the potential).
+        grid_points = 0 evaluates the network inline at every site; > 0 evaluates it once per
+        step on that many points of its compressed rho coordinate and interpolates."""
         if not self.complex:
             raise TypeError("step_mlp needs a complex lattice")
         p = StepParams(dt, dx, eta_eff, mu, inv_a2, noise_scale)
@@ -104,7 +106,7 @@ class Lattice:
         fm = surrogate.spec.features()
         _lib.call("tip_lattice_step_mlp", self._h, C.byref(p), surrogate.params.ptr, C.byref(fm),
                   float(T_start), float(dT_per_step), int(nsteps), s0 & 0xFFFFFFFFFFFFFFFF,
-                  int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(ms) if timed else None)
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, int(grid_points), C.byref(ms) if timed else None)
         self.step_count = s0 + int(nsteps)
         return float(ms.value) if timed else None
 
