@@ -77,6 +77,7 @@ class RunConfig:
     divergence: float = 1e6  # |phi|max threshold of the reference's guard (0 disables)
     guard_every: int = 100000  # latticeSimeRescale_numba.py:3883
     quiet: bool = False
+    progress: int = 0        # log a progress line every this many steps (0 = off)
 
     def __post_init__(self):
         cplx = self.field == "complex"
@@ -247,11 +248,11 @@ class Run:
 
     # -- main loop -----------------------------------------------------------------------
     def _chunk(self, n: int) -> int:
-        """Steps until the next report / guard / snapshot boundary (surrogate path: the
-        temperature is linear in n inside a chunk, so one C call runs it)."""
+        """Steps until the next report / guard / snapshot / progress boundary (surrogate path:
+        the temperature is linear in n inside a chunk, so one C call runs it)."""
         cfg = self.cfg
         k = cfg.Nt - n
-        for every in (cfg.report, cfg.guard_every, cfg.steps if cfg.outdir else 0):
+        for every in (cfg.report, cfg.guard_every, cfg.steps if cfg.outdir else 0, cfg.progress):
             if every:
                 k = min(k, every - n % every)
         return max(k, 1)
@@ -291,6 +292,11 @@ class Run:
                 self.observe(n, T_now)
             if n % min(cfg.guard_every, cfg.report or cfg.guard_every) == 0:
                 self.guard(n, T_now)
+            if cfg.progress and n % cfg.progress == 0:
+                el = time.perf_counter() - t0
+                f = self.series["f_true"][-1] if self.series["f_true"] else float("nan")
+                print(f"[run] step {n}/{cfg.Nt} T={T_now:.2f} {1e3 * el / n:.3f} ms/step "
+                      f"f_true={f:.4f}", flush=True)
             if cfg.steps and cfg.outdir and n % cfg.steps == 0:
                 self.snapshot(n, T_now)
                 el = time.perf_counter() - t0
