@@ -54,7 +54,7 @@ def lattice_candidates(complex_field: bool = False) -> List[Tuple[int, int, int,
 def ensemble_candidates() -> List[Tuple[int, int, int, int, int]]:
     """(rows, planes, xcd, persistent, kernel) over row tiles (kernel 1) and flat tiles (2)."""
     out = []
-    for ty, cx, pers in itertools.product((1, 2, 4), (8, 16, 32), (0, 2048)):
+    for ty, cx, pers in itertools.product((1, 2, 4), (8, 16, 32, 64), (0, 2048)):
         out.append((ty, cx, 1, pers, 1))
     for by, cx, pers in itertools.product((2, 4, 8), (8, 16, 32), (0, 2048)):
         out.append((by, cx, 1, pers, 2))
