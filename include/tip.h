@@ -160,6 +160,74 @@ int tip_rk2_fused_c32(const float *phi1, const float *phi2, const float *pi1, co
                       uint64_t noise_seed, double noise_scale, unsigned long long *vac_count,
                       double vac_esc, void *stream);
 
+/* ---- the reference's other two table integrators ---------------------------------------
+ * Overdamped Euler-Maruyama.  Replaces: overdamped_euler_step_table(phi, dt, dx, eta_eff, mu,
+ * inv_a2, vp_table, vp_tmin, vp_dinv, vp_npts, noise_scale, seed),
+ * simulation/latticeSimeRescale_numba.py:2115-2192 (complex (phi1, phi2, ...):
+ * simulation/latticeSimComplex_numba.py:2422-2517):
+ *   phi_out = phi + (dt / eta_eff) (inv_a2 lap(phi) - V'(phi) / mu^2) + noise_scale z.
+ * BAOAB.  Replaces: baoab_step_table(phi, pi, dt, dx, gamma, mu, inv_a2, vp_table, vp_tmin,
+ * vp_dinv, vp_npts, noise_scale, seed), simulation/latticeSimeRescale_numba.py:1974-2112
+ * (complex: simulation/latticeSimComplex_numba.py:2223-2419), as stages:
+ *   TIP_BAOAB_OPEN   B A O A from the true momentum: phi_out, pi_out = post-O momentum
+ *   TIP_BAOAB_FUSED  the closing half-kick of the previous step (with inv_a2_prev) then OPEN
+ *   TIP_BAOAB_CLOSE  the closing half-kick only: pi_out = pi + dt/2 F(phi) (phi_out unused)
+ * One step is OPEN then CLOSE; k steps are OPEN, FUSED x (k-1), CLOSE (k + 1 lattice passes).
+ * O: pi = c1 pi + c2 z, c1 = exp(-gamma dt), c2 = sqrt(1 - c1^2) noise_scale.
+ * Both evaluate every force on the input field (the reference updates phi in place inside its
+ * parallel loop, so part of each Laplacian sees advanced neighbours); phi_out must not alias
+ * phi, pi_out may alias pi.  Noise: the hash draw of site s = uint64(s * 73856093) ^
+ * noise_seed (z = the cos member; complex (cos, sin)); noise_scale = 0 switches it off.
+ * The reference's seed argument is the step index, so noise_seed = tip_noise_seed(step, 0).
+ * vp_table: device float64 table (the float32 entry points convert it to segments in a
+ * per-device scratch); vp_odd: real tables tabulated on phi >= 0 and read at |phi|. */
+#define TIP_BAOAB_OPEN 1
+#define TIP_BAOAB_FUSED 2
+#define TIP_BAOAB_CLOSE 3
+int tip_overdamped_f32(const float *phi, float *phi_out, double dt, double dx, double eta_eff,
+                       double mu, double inv_a2, const double *vp_table, double vp_tmin,
+                       double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                       int64_t nz, uint64_t noise_seed, double noise_scale, void *stream);
+int tip_overdamped_f64(const double *phi, double *phi_out, double dt, double dx, double eta_eff,
+                       double mu, double inv_a2, const double *vp_table, double vp_tmin,
+                       double vp_dinv, int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                       int64_t nz, uint64_t noise_seed, double noise_scale, void *stream);
+int tip_baoab_f32(int stage, const float *phi, const float *pi, float *phi_out, float *pi_out,
+                  double dt, double dx, double gamma, double mu, double inv_a2,
+                  double inv_a2_prev, const double *vp_table, double vp_tmin, double vp_dinv,
+                  int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny, int64_t nz,
+                  uint64_t noise_seed, double noise_scale, void *stream);
+int tip_baoab_f64(int stage, const double *phi, const double *pi, double *phi_out,
+                  double *pi_out, double dt, double dx, double gamma, double mu, double inv_a2,
+                  double inv_a2_prev, const double *vp_table, double vp_tmin, double vp_dinv,
+                  int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny, int64_t nz,
+                  uint64_t noise_seed, double noise_scale, void *stream);
+int tip_overdamped_c32(const float *phi1, const float *phi2, float *phi1_out, float *phi2_out,
+                       double dt, double dx, double eta_eff, double mu, double inv_a2,
+                       const double *vp_table, double vp_tmin, double vp_dinv, int64_t vp_npts,
+                       double zn_n, double zn_dv, double zn_active, int64_t nx, int64_t ny,
+                       int64_t nz, uint64_t noise_seed, double noise_scale, void *stream);
+int tip_overdamped_c64(const double *phi1, const double *phi2, double *phi1_out,
+                       double *phi2_out, double dt, double dx, double eta_eff, double mu,
+                       double inv_a2, const double *vp_table, double vp_tmin, double vp_dinv,
+                       int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
+                       int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                       void *stream);
+int tip_baoab_c32(int stage, const float *phi1, const float *phi2, const float *pi1,
+                  const float *pi2, float *phi1_out, float *phi2_out, float *pi1_out,
+                  float *pi2_out, double dt, double dx, double gamma, double mu, double inv_a2,
+                  double inv_a2_prev, const double *vp_table, double vp_tmin, double vp_dinv,
+                  int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
+                  int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                  void *stream);
+int tip_baoab_c64(int stage, const double *phi1, const double *phi2, const double *pi1,
+                  const double *pi2, double *phi1_out, double *phi2_out, double *pi1_out,
+                  double *pi2_out, double dt, double dx, double gamma, double mu, double inv_a2,
+                  double inv_a2_prev, const double *vp_table, double vp_tmin, double vp_dinv,
+                  int64_t vp_npts, double zn_n, double zn_dv, double zn_active, int64_t nx,
+                  int64_t ny, int64_t nz, uint64_t noise_seed, double noise_scale,
+                  void *stream);
+
 /* ---- observables ---------------------------------------------------------------------
  * Winding.  Replaces: compute_winding_number(phi1, phi2, winding_out),
  * simulation/latticeSimComplex_numba.py:2683-2744 (numpy restatement tools/winding.py:26-64),
@@ -171,8 +239,10 @@ int tip_rk2_fused_c32(const float *phi1, const float *phi2, const float *pi1, co
  *   counts uint64 [2]          += {#sites with |wsum| > 0.5, #plaquettes with winding != 0}
  * The integers are computed exactly from the signs of the float32 field (the phase
  * difference crosses the branch cut iff the quadrant test and the sign of the exact float64
- * cross product say so); they agree with the reference's float64 atan2 arithmetic except
- * where a phase difference lies within rounding of +-pi. */
+ * cross product say so); edges whose cross product is exactly zero (zero vectors with signed
+ * zeros, the two sides of the negative real axis, antiparallel pairs) take the reference's
+ * float64 atan2 expression.  They agree with the reference's arithmetic except where a phase
+ * difference of a non-degenerate edge lies within rounding of +-pi. */
 int tip_winding_f32(const float *phi1, const float *phi2, int8_t *plaq, float *wsum,
                     unsigned long long *counts, int64_t nx, int64_t ny, int64_t nz,
                     void *stream);
@@ -195,6 +265,12 @@ int tip_vacuum_count_c32(const float *phi1, const float *phi2, int64_t n, double
  * max(rho) of latticeSimComplex_numba.py:4268 (tip_rho_max_c32). */
 int tip_minmax_f32(const float *phi, int64_t n, double *out2, void *stream);
 int tip_minmax_f64(const double *phi, int64_t n, double *out2, void *stream);
+/* Snapshot quantities of the complex driver (latticeSimComplex_numba.py:4625-4626, 4638-4646):
+ * rho = sqrt(phi1^2 + phi2^2) and theta = atan2(phi2, phi1), both formed in float64 from the
+ * float32 field and rounded to float32 (rho, theta: device float[n], nullable), and
+ * {min, max} of the float64 rho (rho_minmax: device double[2], nullable). */
+int tip_rho_theta_f32(const float *phi1, const float *phi2, int64_t n, float *rho, float *theta,
+                      double *rho_minmax, void *stream);
 int tip_rho_max_c32(const float *phi1, const float *phi2, int64_t n, double *out1,
                     void *stream);
 
@@ -243,9 +319,26 @@ int tip_lattice_set_geometry(tip_lattice *lat, int kernel, int rows, int planes_
  * lattice's stream and makes the call synchronous. */
 int tip_lattice_step(tip_lattice *lat, const tip_step_params *p, int64_t nsteps, uint64_t step0,
                      uint64_t seed, float *elapsed_ms);
+/* The overdamped and BAOAB integrators on the lattice (tip_overdamped_*, tip_baoab_*), with
+ * eta_eff as eta (overdamped) or gamma (BAOAB) and noise_scale as the integrator's own
+ * amplitude: overdamped sqrt(2 T dt / (eta_eff mu^2 dx^3)), BAOAB sqrt(T / (mu^2 dx^3))
+ * (latticeSimComplex_numba.py:4293, 4313).  A BAOAB call of nsteps runs OPEN, FUSED x
+ * (nsteps - 1), CLOSE, so pi is the true momentum between calls; the overdamped step leaves
+ * pi untouched. */
+int tip_lattice_step_overdamped(tip_lattice *lat, const tip_step_params *p, int64_t nsteps,
+                                uint64_t step0, uint64_t seed, float *elapsed_ms);
+int tip_lattice_step_baoab(tip_lattice *lat, const tip_step_params *p, int64_t nsteps,
+                           uint64_t step0, uint64_t seed, float *elapsed_ms);
 /* Synchronous observables on the current state. */
 int tip_lattice_vacuum_count(tip_lattice *lat, double esc, unsigned long long *count);
-int tip_lattice_winding(tip_lattice *lat, int8_t *plaq_dev, unsigned long long *counts2);
+/* plaq_dev (int8 [3][N]) and wsum_dev (float [N], the per-site winding density) are nullable
+ * device outputs; counts2 (host) receives {string sites, nonzero plaquettes}. */
+int tip_lattice_winding(tip_lattice *lat, int8_t *plaq_dev, float *wsum_dev,
+                        unsigned long long *counts2);
+/* rho, theta (device float[N], nullable) and {min, max} of rho (host double[2], nullable) of
+ * the current complex field, as tip_rho_theta_f32. */
+int tip_lattice_rho_theta(tip_lattice *lat, float *rho_dev, float *theta_dev,
+                          double *rho_minmax_host);
 int tip_lattice_minmax(tip_lattice *lat, double *out2);
 
 /* ---- batched ensemble ---------------------------------------------------------------

@@ -11,6 +11,8 @@ Reference anchors (paths relative to the reference checkout):
   simulation/latticeSimComplex_numba.py:2067-2215    rk2_step_table (complex)
   simulation/latticeSimeRescale_numba.py:2322-2336   _hash_rng_pair
   simulation/latticeSimComplex_numba.py:2683-2744    compute_winding_number
+  simulation/latticeSimeRescale_numba.py:1974-2192   baoab_step_table, overdamped_euler_step_table
+  simulation/latticeSimComplex_numba.py:2223-2517    (complex forms of the two)
   postprocess/compute_lattice_Tc1.py:24              vacuum fraction
 """
 from __future__ import annotations
@@ -72,6 +74,12 @@ def lib():
         L.orc_rk2_step_table_c32mixed.argtypes = [
             _f32p, _f32p, _f32p, _f32p, _d, _d, _d, _d, _nf32, _nf32, _f32p, _f32p, _f32p, _f32p,
             _d, _f64p, _d, _d, _i64, _d, _d, _d, _i64, _i64, _i64]
+        L.orc_langevin_f64.argtypes = [
+            _int, _f64p, _nf64, _nf64, _nf64, _d, _d, _d, _d, _d, _d, _f64p, _d, _d, _i64, _int,
+            _u64, _d, _i64, _i64, _i64]
+        L.orc_langevin_c64.argtypes = [
+            _int, _f64p, _f64p, _nf64, _nf64, _nf64, _nf64, _nf64, _nf64, _d, _d, _d, _d, _d, _d,
+            _f64p, _d, _d, _i64, _d, _d, _d, _u64, _d, _i64, _i64, _i64]
         L.orc_winding.argtypes = [_f64p, _f64p, _nf32, _ni8, _i64, _i64, _i64]
         L.orc_string_count.argtypes = [_f32p, _i64]
         L.orc_string_count.restype = _i64
@@ -201,6 +209,52 @@ def rk2_step_table_complex(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, no
     fn(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, noise2, phi1_tmp, phi2_tmp, pi1_tmp,
        pi2_tmp, inv_a2, t, vp_tmin, vp_dinv, vp_npts, float(zn[0]), float(zn[1]), float(zn[2]),
        nx, ny, nz)
+
+
+# Overdamped Euler-Maruyama and BAOAB (tip_oracle.c: orc_langevin_f64 / orc_langevin_c64).
+# op: 0 overdamped, 1 BAOAB open (B A O A), 2 fused (closing half-kick + B A O A), 3 close.
+OVERDAMPED, BAOAB_OPEN, BAOAB_FUSED, BAOAB_CLOSE = 0, 1, 2, 3
+
+
+def langevin(op, phi, pi, dt, dx, eta_or_gamma, mu, inv_a2, table, tmin, dinv, odd=False,
+             noise_seed=0, noise_scale=0.0, inv_a2_prev=None):
+    """One stage on float64 copies; returns (phi_out, pi_out) (None where the stage does not
+    write)."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    pi = None if pi is None else np.ascontiguousarray(pi, dtype=np.float64)
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    nx, ny, nz = phi.shape
+    po = np.empty_like(phi) if op != BAOAB_CLOSE else None
+    qo = np.empty_like(phi) if op != OVERDAMPED else None
+    lib().orc_langevin_f64(op, phi, pi, po, qo, dt, dx, eta_or_gamma, mu, inv_a2,
+                           inv_a2 if inv_a2_prev is None else inv_a2_prev, t, tmin, dinv, t.size,
+                           int(odd), noise_seed, noise_scale, nx, ny, nz)
+    return po, qo
+
+
+def langevin_complex(op, phi1, phi2, pi1, pi2, dt, dx, eta_or_gamma, mu, inv_a2, table, tmin,
+                     dinv, zn=(0.0, 0.0, 0.0), noise_seed=0, noise_scale=0.0, inv_a2_prev=None):
+    """Complex-field stage; returns (phi1_out, phi2_out, pi1_out, pi2_out)."""
+    f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    phi1, phi2, pi1, pi2 = f(phi1), f(phi2), f(pi1), f(pi2)
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    nx, ny, nz = phi1.shape
+    mk = lambda c: np.empty_like(phi1) if c else None
+    o = [mk(op != BAOAB_CLOSE), mk(op != BAOAB_CLOSE), mk(op != OVERDAMPED), mk(op != OVERDAMPED)]
+    lib().orc_langevin_c64(op, phi1, phi2, pi1, pi2, *o, dt, dx, eta_or_gamma, mu, inv_a2,
+                           inv_a2 if inv_a2_prev is None else inv_a2_prev, t, tmin, dinv, t.size,
+                           float(zn[0]), float(zn[1]), float(zn[2]), noise_seed, noise_scale,
+                           nx, ny, nz)
+    return tuple(o)
+
+
+def baoab_step(phi, pi, dt, dx, gamma, mu, inv_a2, table, tmin, dinv, odd=False, noise_seed=0,
+               noise_scale=0.0):
+    """One whole BAOAB step (open then close), float64 copies."""
+    p1, q1 = langevin(BAOAB_OPEN, phi, pi, dt, dx, gamma, mu, inv_a2, table, tmin, dinv, odd,
+                      noise_seed, noise_scale)
+    _, q2 = langevin(BAOAB_CLOSE, p1, q1, dt, dx, gamma, mu, inv_a2, table, tmin, dinv, odd)
+    return p1, q2
 
 
 def run_real(phi, pi, nsteps, params, table, tmin, dinv, odd=False, seed=0, step0=0,

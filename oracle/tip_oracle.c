@@ -493,3 +493,151 @@ ORC_API int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Overdamped Euler-Maruyama and BAOAB, float64, race-free (Jacobi) form.
+ *   overdamped_euler_step_table: real  simulation/latticeSimeRescale_numba.py:2115-2192,
+ *                                complex simulation/latticeSimComplex_numba.py:2422-2517
+ *   baoab_step_table:            real  simulation/latticeSimeRescale_numba.py:1974-2112,
+ *                                complex simulation/latticeSimComplex_numba.py:2223-2419
+ * The reference writes phi in place inside its parallel loop, so some neighbours in a site's
+ * Laplacian are already advanced (schedule dependent).  Here every force is evaluated on one
+ * field: the stencil reads phi, the result goes to phi_out.  Per-site arithmetic in the
+ * reference's textual order:
+ *   force = inv_a2 * lap - dV * inv_mu2
+ *   overdamped: phi_out = ph + dt_over_eta * force + noise_scale * z          (:2192)
+ *   BAOAB stage OPEN (1):  pi_v = pi + half_dt * force; ph_a = ph + half_dt * pi_v;  (:2058-2060)
+ *                          pi_b = c1 * pi_v + c2 * z;                               (:2073)
+ *                          phi_out = ph_a + half_dt * pi_b; pi_out = pi_b           (:2094)
+ *   stage CLOSE (3):       pi_out = pi + half_dt * force_prev                       (:2112)
+ *   stage FUSED (2):       CLOSE with force_prev, then OPEN on the result
+ * force_prev uses inv_a2_prev (the previous step's scale factor).  c1 = exp(-gamma dt),
+ * c2 = sqrt(1 - c1 c1) noise_scale (:2008-2009).  z: the hash pair of the site
+ * (mult 73856093); the real field takes the cos member, the complex field (cos, sin).
+ * ------------------------------------------------------------------------------------------ */
+static inline void langevin_site(int op, double ph, double p, double F, double Fp, double z,
+                                 double half_dt, double dt_over_eta, double c1, double c2,
+                                 double ns, double *ph_out, double *p_out)
+{
+    if (op == 0) {
+        *ph_out = ph + dt_over_eta * F + ns * z;
+        return;
+    }
+    if (op == 2 || op == 3) p = p + half_dt * Fp;
+    if (op == 3) {
+        *p_out = p;
+        *ph_out = ph;
+        return;
+    }
+    double pv = p + half_dt * F;
+    double pa = ph + half_dt * pv;
+    double pb = c1 * pv + c2 * z;
+    *ph_out = pa + half_dt * pb;
+    *p_out = pb;
+}
+
+ORC_API void orc_langevin_f64(int op, const double *phi, const double *pi, double *phi_out,
+                              double *pi_out, double dt, double dx, double eta_or_gamma,
+                              double mu, double inv_a2, double inv_a2_prev, const double *table,
+                              double tmin, double dinv, int64_t npts, int odd,
+                              uint64_t noise_seed, double noise_scale, int64_t nx, int64_t ny,
+                              int64_t nz)
+{
+    const double inv_dx2 = 1.0 / (dx * dx);
+    const double inv_mu2 = 1.0 / (mu * mu);
+    const double half_dt = 0.5 * dt;
+    const double dt_over_eta = dt / eta_or_gamma;
+    const double c1 = exp(-eta_or_gamma * dt);
+    const double c2 = sqrt(1.0 - c1 * c1) * noise_scale;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nx; ++i) {
+        int64_t ip = i + 1 < nx ? i + 1 : 0, im = i - 1 >= 0 ? i - 1 : nx - 1;
+        for (int64_t j = 0; j < ny; ++j) {
+            int64_t jp = j + 1 < ny ? j + 1 : 0, jm = j - 1 >= 0 ? j - 1 : ny - 1;
+            for (int64_t k = 0; k < nz; ++k) {
+                int64_t kp = k + 1 < nz ? k + 1 : 0, km = k - 1 >= 0 ? k - 1 : nz - 1;
+                int64_t s = IDX(i, j, k);
+                double ph = phi[s];
+                double lap = (phi[IDX(ip, j, k)] + phi[IDX(im, j, k)] + phi[IDX(i, jp, k)] +
+                              phi[IDX(i, jm, k)] + phi[IDX(i, j, kp)] + phi[IDX(i, j, km)] -
+                              6.0 * ph) * inv_dx2;
+                double dV = lookup(ph, table, tmin, dinv, npts, odd);
+                double F = inv_a2 * lap - dV * inv_mu2;
+                double Fp = inv_a2_prev * lap - dV * inv_mu2;
+                double z = 0.0;
+                if (op != 3 && noise_scale != 0.0) {
+                    uint64_t x1, x2;
+                    double u1, u2, zs;
+                    hash_pair(site_seed(s, 73856093, noise_seed), &x1, &x2, &u1, &u2);
+                    box_muller(u1, u2, &z, &zs);
+                }
+                double a, p;
+                langevin_site(op, ph, op == 0 ? 0.0 : pi[s], F, Fp, z, half_dt, dt_over_eta,
+                              c1, c2, noise_scale, &a, &p);
+                if (op != 3) phi_out[s] = a;
+                if (op != 0) pi_out[s] = p;
+            }
+        }
+    }
+}
+
+ORC_API void orc_langevin_c64(int op, const double *phi1, const double *phi2, const double *pi1,
+                              const double *pi2, double *phi1_out, double *phi2_out,
+                              double *pi1_out, double *pi2_out, double dt, double dx,
+                              double eta_or_gamma, double mu, double inv_a2, double inv_a2_prev,
+                              const double *table, double tmin, double dinv, int64_t npts,
+                              double zn_n, double zn_dv, double zn_active, uint64_t noise_seed,
+                              double noise_scale, int64_t nx, int64_t ny, int64_t nz)
+{
+    const double inv_dx2 = 1.0 / (dx * dx);
+    const double inv_mu2 = 1.0 / (mu * mu);
+    const double half_dt = 0.5 * dt;
+    const double dt_over_eta = dt / eta_or_gamma;
+    const double c1 = exp(-eta_or_gamma * dt);
+    const double c2 = sqrt(1.0 - c1 * c1) * noise_scale;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nx; ++i) {
+        int64_t ip = i + 1 < nx ? i + 1 : 0, im = i - 1 >= 0 ? i - 1 : nx - 1;
+        for (int64_t j = 0; j < ny; ++j) {
+            int64_t jp = j + 1 < ny ? j + 1 : 0, jm = j - 1 >= 0 ? j - 1 : ny - 1;
+            for (int64_t k = 0; k < nz; ++k) {
+                int64_t kp = k + 1 < nz ? k + 1 : 0, km = k - 1 >= 0 ? k - 1 : nz - 1;
+                int64_t s = IDX(i, j, k);
+                double p1 = phi1[s], p2 = phi2[s];
+                double rho = sqrt(p1 * p1 + p2 * p2);
+                double lap1 = (phi1[IDX(ip, j, k)] + phi1[IDX(im, j, k)] + phi1[IDX(i, jp, k)] +
+                               phi1[IDX(i, jm, k)] + phi1[IDX(i, j, kp)] + phi1[IDX(i, j, km)] -
+                               6.0 * p1) * inv_dx2;
+                double lap2 = (phi2[IDX(ip, j, k)] + phi2[IDX(im, j, k)] + phi2[IDX(i, jp, k)] +
+                               phi2[IDX(i, jm, k)] + phi2[IDX(i, j, kp)] + phi2[IDX(i, j, km)] -
+                               6.0 * p2) * inv_dx2;
+                double dV1, dV2;
+                complex_force(p1, p2, table, tmin, dinv, npts, zn_n, zn_dv, zn_active, rho, &dV1,
+                              &dV2);
+                double F1 = inv_a2 * lap1 - dV1 * inv_mu2, F2 = inv_a2 * lap2 - dV2 * inv_mu2;
+                double F1p = inv_a2_prev * lap1 - dV1 * inv_mu2;
+                double F2p = inv_a2_prev * lap2 - dV2 * inv_mu2;
+                double z1 = 0.0, z2 = 0.0;
+                if (op != 3 && noise_scale != 0.0) {
+                    uint64_t x1, x2;
+                    double u1, u2;
+                    hash_pair(site_seed(s, 73856093, noise_seed), &x1, &x2, &u1, &u2);
+                    box_muller(u1, u2, &z1, &z2);
+                }
+                double a, b, u, v;
+                langevin_site(op, p1, op == 0 ? 0.0 : pi1[s], F1, F1p, z1, half_dt, dt_over_eta,
+                              c1, c2, noise_scale, &a, &u);
+                langevin_site(op, p2, op == 0 ? 0.0 : pi2[s], F2, F2p, z2, half_dt, dt_over_eta,
+                              c1, c2, noise_scale, &b, &v);
+                if (op != 3) {
+                    phi1_out[s] = a;
+                    phi2_out[s] = b;
+                }
+                if (op != 0) {
+                    pi1_out[s] = u;
+                    pi2_out[s] = v;
+                }
+            }
+        }
+    }
+}

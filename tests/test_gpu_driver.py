@@ -83,3 +83,58 @@ def test_surrogate_chunked_equals_stepwise():
             run.close()
     for a, b in zip(*out):
         assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 64), (5, 7, 300)])
+def test_save_point_on_device(shape):
+    """The complex save point computed on the device (latticeSimComplex_numba.py:4625-4646):
+    rho and its range bit-exact against numpy's float64 formula, theta within one float32 ulp
+    of numpy's arctan2, the winding buffer equal to the oracle's rounded site sums."""
+    from tip.lattice import Lattice
+    rng = np.random.default_rng(shape[2])
+    p1 = (rng.normal(size=shape) * 300.0).astype(np.float32)
+    p2 = (rng.normal(size=shape) * 300.0).astype(np.float32)
+    p1.ravel()[:4] = [0.0, -0.0, 0.0, 1e-30]
+    p2.ravel()[:4] = [0.0, 0.0, -0.0, -1e-30]
+    lat = Lattice(shape, np.float32, complex_field=True)
+    try:
+        lat.upload("phi1", p1)
+        lat.upload("phi2", p2)
+        sp = lat.save_point()
+    finally:
+        lat.close()
+    rho = np.sqrt(p1.astype(np.float64) ** 2 + p2.astype(np.float64) ** 2)
+    theta = np.arctan2(p2.astype(np.float64), p1.astype(np.float64)).astype(np.float32)
+    assert np.array_equal(sp["rho"], rho.astype(np.float32))
+    assert sp["rho_min"] == float(rho.min()) and sp["rho_max"] == float(rho.max())
+    ulp = np.abs(sp["theta"].view(np.int32).astype(np.int64) - theta.view(np.int32))
+    assert int(ulp.max()) <= 1
+    assert np.array_equal(np.signbit(sp["theta"]), np.signbit(theta))
+    w_ref, _ = O.winding(p1, p2)
+    assert np.array_equal(sp["winding"], np.rint(w_ref).astype(np.float32))
+    assert sp["n_strings"] == O.string_count(w_ref)
+
+
+def test_driver_snapshot_keys(tmp_path):
+    """A complex run with snapshots writes the reference's npz keys; the derived arrays agree
+    with the saved fields."""
+    from tip import driver as D
+    cfg = D.RunConfig(field="complex", Nx=16, Ny=16, Nz=64, Nt=20, T0=7350.0, report=10,
+                      steps=10, seed=2, quiet=True, outdir=str(tmp_path), init_rho=50.0)
+    run = D.Run(cfg)
+    try:
+        run.run()
+    finally:
+        run.close()
+    files = sorted(tmp_path.glob("state_step_*.npz"))
+    assert len(files) >= 2
+    with np.load(files[-1]) as z:
+        keys = set(z.files)
+        assert {"phi1", "phi2", "rho", "theta", "winding", "step", "time", "temperature",
+                "rho_min", "rho_max", "zn_active"} <= keys
+        p1, p2 = z["phi1"], z["phi2"]
+        rho = np.sqrt(p1.astype(np.float64) ** 2 + p2.astype(np.float64) ** 2)
+        assert np.array_equal(z["rho"], rho.astype(np.float32))
+        assert float(z["rho_max"]) == float(rho.max())
+        w_ref, _ = O.winding(p1, p2)
+        assert np.array_equal(z["winding"], np.rint(w_ref).astype(np.float32))

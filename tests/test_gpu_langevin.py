@@ -1,0 +1,249 @@
+"""Overdamped Euler-Maruyama and BAOAB on the GPU (-m gpu) against the CPU oracle.
+
+Reference: overdamped_euler_step_table / baoab_step_table,
+simulation/latticeSimeRescale_numba.py:1974-2192 and simulation/latticeSimComplex_numba.py:
+2223-2517, in the race-free form the oracle restates (oracle/tip_oracle.c orc_langevin_*).
+
+Bars:
+  * float64 drop-ins (real; complex with Z_N on): fields within 1e-12 relative of the oracle
+    after 6 steps with hash noise (the only differences are libm's log/cos/sin/atan2)
+  * float32 drop-ins: within 2e-5 relative of the float64 oracle after 6 steps
+  * lattice BAOAB over k steps (open, fused x (k-1), close): float64 bit-identical to k
+    drop-in steps; float32 within 1e-6 relative; the overdamped lattice path equals the
+    drop-in loop and leaves pi untouched
+  * the driver's complex overdamped and BAOAB loops replay on the oracle fed the same tables
+    (float32 bar 2e-5)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(12, 10, 70), (5, 7, 33)]
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def real_setup(shape, seed):
+    from tip import potential as P
+    rng = np.random.default_rng(seed)
+    p = P.ModelParams.real()
+    vt = P.thermal_vev(7000.0, p)
+    t, tmin, dinv = P.build_vprime_table(7000.0, 0.0, 1.5 * vt, 4096, p)
+    phi = rng.normal(size=shape) * 0.3 * vt
+    pi = rng.normal(size=shape) * 50.0
+    # dt, dx, eta / gamma, mu, inv_a2, noise_scale
+    prm = dict(dt=0.25, dx=2.0, eta=7.0, mu=1000.0, inv_a2=0.93, ns=40.0)
+    return phi, pi, (t, tmin, dinv), prm
+
+
+def complex_setup(shape, seed):
+    from tip import potential as P
+    rng = np.random.default_rng(seed)
+    t, tmin, dinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, P.ModelParams.complex())
+    f = [rng.normal(size=shape) * 2e3 for _ in range(2)]
+    q = [rng.normal(size=shape) * 1e2 for _ in range(2)]
+    prm = dict(dt=0.25, dx=1.0, eta=1.3, mu=1000.0, inv_a2=1.0, ns=300.0)
+    return f, q, (t, tmin, dinv), prm
+
+
+def oracle_real(op, phi, pi, tab, prm, nsteps, odd=True):
+    for n in range(nsteps):
+        kw = dict(noise_seed=O.noise_seed(n), noise_scale=prm["ns"])
+        if op == "overdamped":
+            phi, _ = O.langevin(O.OVERDAMPED, phi, None, prm["dt"], prm["dx"], prm["eta"],
+                                prm["mu"], prm["inv_a2"], *tab, odd=odd, **kw)
+        else:
+            phi, pi = O.baoab_step(phi, pi, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                   prm["inv_a2"], *tab, odd=odd, **kw)
+    return phi, pi
+
+
+def oracle_complex(op, f, q, tab, prm, zn, nsteps):
+    a, b = f
+    u, v = q
+    for n in range(nsteps):
+        kw = dict(zn=zn, noise_seed=O.noise_seed(n), noise_scale=prm["ns"])
+        args = (prm["dt"], prm["dx"], prm["eta"], prm["mu"], prm["inv_a2"], *tab)
+        if op == "overdamped":
+            a, b, _, _ = O.langevin_complex(O.OVERDAMPED, a, b, None, None, *args, **kw)
+        else:
+            a, b, u, v = O.langevin_complex(O.BAOAB_OPEN, a, b, u, v, *args, **kw)
+            _, _, u, v = O.langevin_complex(O.BAOAB_CLOSE, a, b, u, v, *args[:5], *tab, zn=zn)
+    return (a, b), (u, v)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("op", ["overdamped", "baoab"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_real_drop_in_matches_oracle(shape, op, dtype):
+    from tip import kernels as K
+    phi0, pi0, tab, prm = real_setup(shape, 3)
+    phi0, pi0 = phi0.astype(dtype), pi0.astype(dtype)
+    phi, pi = phi0.copy(), pi0.copy()
+    for n in range(6):
+        if op == "overdamped":
+            K.overdamped_euler_step_table(phi, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                          prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]),
+                                          prm["ns"], O.noise_seed(n), odd=True)
+        else:
+            K.baoab_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                               prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]), prm["ns"],
+                               O.noise_seed(n), odd=True)
+    rp, rq = oracle_real(op, phi0.astype(np.float64), pi0.astype(np.float64), tab, prm, 6)
+    bar = 1e-12 if dtype == np.float64 else 2e-5
+    assert rel(phi.astype(np.float64), rp) < bar
+    if op == "baoab":
+        assert rel(pi.astype(np.float64), rq) < bar
+    else:
+        assert np.array_equal(pi, pi0)
+
+
+@pytest.mark.parametrize("op", ["overdamped", "baoab"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_complex_drop_in_matches_oracle(op, dtype):
+    from tip import kernels_complex as KC
+    shape = (10, 9, 70)
+    f, q, tab, prm = complex_setup(shape, 5)
+    zn = (3.0, 1e8, 1.0)
+    a, b = (x.astype(dtype) for x in f)
+    u, v = (x.astype(dtype) for x in q)
+    f0 = [a.astype(np.float64), b.astype(np.float64)]
+    q0 = [u.astype(np.float64), v.astype(np.float64)]
+    for n in range(6):
+        if op == "overdamped":
+            KC.overdamped_euler_step_table(a, b, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                           prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]),
+                                           prm["ns"], O.noise_seed(n), zn=zn)
+        else:
+            KC.baoab_step_table(a, b, u, v, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]), prm["ns"],
+                                O.noise_seed(n), zn=zn)
+    (ra, rb), (ru, rv) = oracle_complex(op, f0, q0, tab, prm, zn, 6)
+    bar = 1e-12 if dtype == np.float64 else 2e-5
+    assert rel(a.astype(np.float64), ra) < bar and rel(b.astype(np.float64), rb) < bar
+    if op == "baoab":
+        assert rel(u.astype(np.float64), ru) < bar and rel(v.astype(np.float64), rv) < bar
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_lattice_baoab_fused_equals_steps(dtype):
+    from tip import kernels as K
+    from tip.lattice import Lattice
+    shape = (16, 12, 70)
+    phi0, pi0, tab, prm = real_setup(shape, 7)
+    phi0, pi0 = phi0.astype(dtype), pi0.astype(dtype)
+    k = 9
+    lat = Lattice(shape, dtype)
+    try:
+        lat.upload("phi", phi0)
+        lat.upload("pi", pi0)
+        lat.set_table(*tab, odd=True)
+        lat.step_baoab(k, dt=prm["dt"], dx=prm["dx"], gamma=prm["eta"], mu=prm["mu"],
+                       inv_a2=prm["inv_a2"], noise_scale=prm["ns"], seed=0, step0=0)
+        lp, lq = lat.download("phi"), lat.download("pi")
+    finally:
+        lat.close()
+    phi, pi = phi0.copy(), pi0.copy()
+    for n in range(k):
+        K.baoab_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta"], prm["mu"], prm["inv_a2"],
+                           tab[0], tab[1], tab[2], len(tab[0]), prm["ns"], O.noise_seed(n),
+                           odd=True)
+    if dtype == np.float64:
+        assert np.array_equal(lp, phi) and np.array_equal(lq, pi)
+    else:
+        assert rel(lp.astype(np.float64), phi.astype(np.float64)) < 1e-6
+        assert rel(lq.astype(np.float64), pi.astype(np.float64)) < 1e-6
+
+
+def test_lattice_complex_integrators_match_drop_in():
+    from tip import kernels_complex as KC
+    from tip.lattice import Lattice
+    shape = (8, 12, 96)
+    f, q, tab, prm = complex_setup(shape, 9)
+    zn = (3.0, 1e8, 1.0)
+    init = dict(phi1=f[0], phi2=f[1], pi1=q[0], pi2=q[1])
+    init = {k: v.astype(np.float32) for k, v in init.items()}
+    for op in ("overdamped", "baoab"):
+        lat = Lattice(shape, np.float32, complex_field=True)
+        try:
+            for k_, v_ in init.items():
+                lat.upload(k_, v_)
+            lat.set_table(*tab)
+            lat.set_zn(*zn)
+            kw = dict(dt=prm["dt"], dx=prm["dx"], mu=prm["mu"], inv_a2=prm["inv_a2"],
+                      noise_scale=prm["ns"], seed=0, step0=0)
+            if op == "baoab":
+                lat.step_baoab(5, gamma=prm["eta"], **kw)
+            else:
+                lat.step_overdamped(5, eta_eff=prm["eta"], **kw)
+            got = {k_: lat.download(k_) for k_ in init}
+        finally:
+            lat.close()
+        a, b, u, v = (init[k_].copy() for k_ in ("phi1", "phi2", "pi1", "pi2"))
+        for n in range(5):
+            if op == "baoab":
+                KC.baoab_step_table(a, b, u, v, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                    prm["inv_a2"], *tab, len(tab[0]), prm["ns"],
+                                    O.noise_seed(n), zn=zn)
+            else:
+                KC.overdamped_euler_step_table(a, b, prm["dt"], prm["dx"], prm["eta"],
+                                               prm["mu"], prm["inv_a2"], *tab, len(tab[0]),
+                                               prm["ns"], O.noise_seed(n), zn=zn)
+        for k_, ref in zip(("phi1", "phi2", "pi1", "pi2"), (a, b, u, v)):
+            assert rel(got[k_].astype(np.float64), ref.astype(np.float64)) < 1e-6, (op, k_)
+        if op == "overdamped":
+            assert np.array_equal(got["pi1"], init["pi1"])
+
+
+@pytest.mark.parametrize("integrator", ["overdamped", "baoab"])
+def test_driver_complex_integrators_replay_on_oracle(integrator):
+    from tip import driver as D
+    cfg = D.RunConfig(field="complex", Nx=16, Ny=16, Nz=64, Nt=12, T0=7350.0, cool_to=7000.0,
+                      report=6, seed=0, quiet=True, init_rho=300.0, integrator=integrator,
+                      table_size=4096, zn_order=3, zn_strength=1e8)
+    run = D.Run(cfg)
+    tables = []
+    orig = run.lat.set_table
+
+    def rec(table, tmin, dinv, odd=False):
+        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
+        orig(table, tmin, dinv, odd=odd)
+
+    run.lat.set_table = rec
+    try:
+        run.run()
+        got = [run.lat.download(k).astype(np.float64) for k in ("phi1", "phi2", "pi1", "pi2")]
+    finally:
+        run.close()
+    f0 = D.initial_fields(cfg)
+    a, b = f0["phi1"].astype(np.float64), f0["phi2"].astype(np.float64)
+    u, v = f0["pi1"].astype(np.float64), f0["pi2"].astype(np.float64)
+    dt, dx, eta = D.MU * cfg.dt_phys, D.MU * cfg.dx_phys, cfg.eta_phys / D.MU
+    k = 0
+    for n in range(cfg.Nt):
+        while k + 1 < len(tables) and tables[k + 1][0] <= n:
+            k += 1
+        _, t, tmin, dinv = tables[k]
+        T = cfg.T0 + (cfg.cool_to - cfg.T0) * n / cfg.Nt
+        zn = (3.0, 1e8, 1.0)
+        if integrator == "overdamped":
+            ns = math.sqrt(2.0 * T * dt / (eta * D.MU ** 2 * cfg.dx_phys ** 3))
+            a, b, _, _ = O.langevin_complex(O.OVERDAMPED, a, b, None, None, dt, dx, eta, D.MU,
+                                            1.0, t, tmin, dinv, zn=zn,
+                                            noise_seed=O.noise_seed(n), noise_scale=ns)
+        else:
+            ns = math.sqrt(T / (D.MU ** 2 * cfg.dx_phys ** 3))
+            a, b, u, v = O.langevin_complex(O.BAOAB_OPEN, a, b, u, v, dt, dx, eta, D.MU, 1.0, t,
+                                            tmin, dinv, zn=zn, noise_seed=O.noise_seed(n),
+                                            noise_scale=ns)
+            _, _, u, v = O.langevin_complex(O.BAOAB_CLOSE, a, b, u, v, dt, dx, eta, D.MU, 1.0,
+                                            t, tmin, dinv, zn=zn)
+    for g, r in zip(got, (a, b, u, v)):
+        assert rel(g, r) < 2e-5

@@ -354,6 +354,21 @@ def test_winding_bit_exact(shape):
     assert np.array_equal(plaq2, plaq_ref) and nstr2 == nstr
 
 
+def test_winding_degenerate_edges():
+    """Edges whose cross product is exactly zero: zero vectors with signed zeros (angle +-pi),
+    the two sides of the negative real axis (theta = pi and -pi) and antiparallel pairs; the
+    kernel falls back to the reference's atan2 expression there."""
+    tip, K, KC, DA, _ = _tip()
+    shape = (6, 8, 70)
+    rng = np.random.default_rng(31)
+    p1 = rng.choice(np.array([-2.0, -1.0, -0.0, 0.0, 1.0, 2.0], np.float32), size=shape)
+    p2 = rng.choice(np.array([-1.0, -0.0, 0.0, 1.0], np.float32), size=shape)
+    w_ref, plaq_ref = O.winding(p1, p2)
+    w, nstr, nplq, plaq = KC.compute_winding_number(p1, p2, plaquettes=True)
+    assert np.array_equal(plaq, plaq_ref)
+    assert nstr == O.string_count(w_ref)
+
+
 def test_winding_vortex_ring():
     """A vortex ring: winding lives on a closed loop of plaquettes; counts match."""
     tip, K, KC, DA, _ = _tip()

@@ -3,9 +3,9 @@
 Usage:  python build.py [--jobs N] [--verbose]
 
 Every .hip file under csrc/ is compiled to an object for gfx950 and linked into
-tip/libtip.so, next to the Python host package that loads it.  The float64 step file is
-compiled with -ffp-contract=off so the float64 drop-in keeps the reference's rounding
-sequence; everything else may contract to FMA.
+tip/libtip.so, next to the Python host package that loads it.  The
+float64 step files are compiled with -ffp-contract=off so the float64 drop-ins keep the
+reference's rounding sequence; everything else may contract to FMA.
 """
 from __future__ import annotations
 
@@ -43,6 +43,7 @@ COMMON = [
 
 PER_FILE = {
     "tip_step_f64.hip": ["-ffp-contract=off"],
+    "tip_langevin_f64.hip": ["-ffp-contract=off"],
 }
 
 

@@ -279,7 +279,8 @@ extern "C" int tip_noise_field_f32(float* noise1, float* noise2, int64_t nx, int
 //           i.e. cross(a, b) = a1 b2 - a2 b1 < 0;
 //   m = -1  iff theta_b < 0 < theta_a and cross(a, b) > 0.
 // For float32 inputs the float64 products are exact and the difference is correctly rounded,
-// so the sign of cross is exact: the kernel needs no transcendental at all.
+// so the sign of cross is exact and decides every edge except the cross == 0 cases below,
+// which take the reference's transcendental expression.
 // ------------------------------------------------------------------------------------------
 template <typename R>
 __device__ __forceinline__ int half_plane(R x, R y)
@@ -295,11 +296,23 @@ __device__ __forceinline__ int half_plane(R x, R y)
     return 0;  // NaN
 }
 
+// cross(a, b) == 0 with a and b in opposite open half planes: a zero vector whose signed zeros
+// give it the angle +-pi, the two sides of the negative real axis (theta = pi and -pi), or an
+// exactly antiparallel pair (|d| = pi up to rounding).  The sign of cross cannot decide these;
+// evaluate the reference's own expression, m = (d - atan2(sin d, cos d)) / 2 pi.
+__device__ __noinline__ int edge_wrap_degenerate(double ax, double ay, double bx, double by)
+{
+    const double d = atan2(by, bx) - atan2(ay, ax);
+    const double w = atan2(sin(d), cos(d));
+    return (int)rint((d - w) * 0.15915494309189535);  // 1 / (2 pi)
+}
+
 template <typename R>
 __device__ __forceinline__ int edge_wrap(R ax, R ay, int ha, R bx, R by, int hb)
 {
     if (ha == hb || ha == 0 || hb == 0) return 0;
     double cr = (double)ax * (double)by - (double)ay * (double)bx;
+    if (cr == 0.0) return edge_wrap_degenerate((double)ax, (double)ay, (double)bx, (double)by);
     if (ha < 0) return cr < 0.0 ? 1 : 0;
     return cr > 0.0 ? -1 : 0;
 }
@@ -465,7 +478,8 @@ extern "C" int tip_vacuum_count_c32(const float* phi1, const float* phi2, int64_
 }
 
 // Two-stage min/max: per-block partials into scratch, then one block folds them.
-// mode 0: min/max of a; mode 1: max of sqrt(a^2 + b^2) (written to out[0]).
+// mode 0: min/max of a; mode 1: max of sqrt(a^2 + b^2) (written to out[0]); mode 2: min/max of
+// sqrt(a^2 + b^2).
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_minmax_part(const R* __restrict__ a,
                                                      const R* __restrict__ b, int64_t n,
@@ -477,7 +491,7 @@ __global__ __launch_bounds__(256) void k_minmax_part(const R* __restrict__ a,
          s += (int64_t)gridDim.x * blockDim.x) {
         double v;
         if (MODE == 0) v = (double)a[s];
-        else {
+        else {  // modes 1, 2: rho
             double u = (double)a[s], w = (double)b[s];
             v = sqrt(u * u + w * w);
         }
@@ -526,7 +540,7 @@ __global__ __launch_bounds__(256) void k_minmax_fold(const double* __restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (mode == 0) {
+        if (mode != 1) {
             out[0] = smin[0];
             out[1] = smax[0];
         } else {
@@ -559,6 +573,35 @@ extern "C" int tip_minmax_f32(const float* phi, int64_t n, double* out2, void* s
 extern "C" int tip_minmax_f64(const double* phi, int64_t n, double* out2, void* stream)
 {
     return minmax<double, 0>(phi, nullptr, n, out2, stream);
+}
+
+// Snapshot fields of the complex driver (latticeSimComplex_numba.py:4625-4626): rho and theta
+// formed in float64 from the float32 field and rounded to float32.
+__global__ __launch_bounds__(256) void k_rho_theta(const float* __restrict__ a,
+                                                   const float* __restrict__ b, int64_t n,
+                                                   float* __restrict__ rho,
+                                                   float* __restrict__ theta)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const double u = (double)a[s], w = (double)b[s];
+        if (rho) rho[s] = (float)sqrt(u * u + w * w);  // u*u, w*w exact: one rounding, then sqrt
+        if (theta) theta[s] = (float)atan2(w, u);
+    }
+}
+
+extern "C" int tip_rho_theta_f32(const float* phi1, const float* phi2, int64_t n, float* rho,
+                                 float* theta, double* rho_minmax, void* stream)
+{
+    TIP_REQUIRE(phi1 && phi2 && n > 0, "tip_rho_theta_f32: bad arguments");
+    hipStream_t st = as_stream(stream);
+    if (rho || theta) {
+        hipLaunchKernelGGL(k_rho_theta, dim3(reduce_blocks(n)), dim3(256), 0, st, phi1, phi2, n,
+                           rho, theta);
+        TIP_HIP(hipGetLastError());
+    }
+    if (rho_minmax) return minmax<float, 2>(phi1, phi2, n, rho_minmax, stream);
+    return TIP_OK;
 }
 
 extern "C" int tip_rho_max_c32(const float* phi1, const float* phi2, int64_t n, double* out1,

@@ -1,0 +1,156 @@
+// tip_langevin_f32.hip -- float32 overdamped Euler-Maruyama and BAOAB steps, real and complex,
+// reading the float32 segment form of the V' table.
+#include "langevin_kernels.h"
+
+using namespace tip;
+
+namespace tip {
+
+int langevin_real_f32(int op, const float* phi, const float* pi, float* phi_out, float* pi_out,
+                      const LangevinScalars<float>& c, const SegTable& tab, int64_t nx,
+                      int64_t ny, int64_t nz, uint64_t noise_seed, int hash, hipStream_t st)
+{
+    if (tab.odd)
+        return launch_langevin_real<float, TableF32<true>>(op, phi, pi, phi_out, pi_out, c,
+                                                           tab.as<true>(), nx, ny, nz,
+                                                           noise_seed, hash, st);
+    return launch_langevin_real<float, TableF32<false>>(op, phi, pi, phi_out, pi_out, c,
+                                                        tab.as<false>(), nx, ny, nz, noise_seed,
+                                                        hash, st);
+}
+
+int langevin_complex_f32(int op, const float* phi1, const float* phi2, const float* pi1,
+                         const float* pi2, float* phi1_out, float* phi2_out, float* pi1_out,
+                         float* pi2_out, const LangevinScalars<float>& c, const SegTable& tab,
+                         const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
+                         uint64_t noise_seed, int hash, hipStream_t st)
+{
+    return launch_langevin_complex<float, TableF32<false>>(op, phi1, phi2, pi1, pi2, phi1_out,
+                                                           phi2_out, pi1_out, pi2_out, c,
+                                                           tab.as<false>(), zn, nx, ny, nz,
+                                                           noise_seed, hash, st);
+}
+
+}  // namespace tip
+
+namespace {
+
+// The float32 kernels read the (A, B) segment form of the float64 device table, prepared into
+// the per-device scratch (slot 0, as the float32 RK2 drop-in).
+int prepared_table(const char* who, int64_t nx, int64_t ny, int64_t nz, const double* vp_table,
+                   int64_t npts, double tmin, double dinv, int odd, double eta, hipStream_t st,
+                   SegTable& out)
+{
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    TIP_REQUIRE(vp_table, "%s: NULL table", who);
+    TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "%s: vp_npts out of range", who);
+    TIP_REQUIRE(eta > 0.0, "%s: eta / gamma must be positive", who);
+    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(npts - 1) * sizeof(float2), 0));
+    TIP_REQUIRE(ab != nullptr, "%s: scratch allocation failed", who);
+    rc = prepare_table_f32(vp_table, npts, tmin, dinv, ab, st);
+    if (rc) return rc;
+    out = make_table32(ab, tmin, dinv, npts, odd);
+    return TIP_OK;
+}
+
+int check_stage(const char* who, int stage, const void* phi, const void* pi, const void* phi_out,
+                const void* pi_out)
+{
+    TIP_REQUIRE(stage == LV_BAOAB_OPEN || stage == LV_BAOAB_FUSED || stage == LV_BAOAB_CLOSE,
+                "%s: stage must be TIP_BAOAB_OPEN, _FUSED or _CLOSE (got %d)", who, stage);
+    TIP_REQUIRE(phi && pi && pi_out, "%s: NULL field", who);
+    if (stage != LV_BAOAB_CLOSE) {
+        TIP_REQUIRE(phi_out, "%s: NULL phi_out", who);
+        TIP_REQUIRE(phi_out != phi, "%s: phi_out must not alias phi", who);
+    }
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_overdamped_f32(const float* phi, float* phi_out, double dt, double dx,
+                                  double eta_eff, double mu, double inv_a2,
+                                  const double* vp_table, double vp_tmin, double vp_dinv,
+                                  int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                                  int64_t nz, uint64_t noise_seed, double noise_scale,
+                                  void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    TIP_REQUIRE(phi && phi_out && phi != phi_out, "overdamped f32: NULL or aliased field");
+    SegTable tab;
+    int rc = prepared_table("overdamped f32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv,
+                            vp_odd, eta_eff, st, tab);
+    if (rc) return rc;
+    return langevin_real_f32(LV_OVERDAMPED, phi, nullptr, phi_out, nullptr,
+                             make_langevin<float>(dt, dx, eta_eff, mu, inv_a2, inv_a2,
+                                                  noise_scale),
+                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st);
+}
+
+extern "C" int tip_baoab_f32(int stage, const float* phi, const float* pi, float* phi_out,
+                             float* pi_out, double dt, double dx, double gamma, double mu,
+                             double inv_a2, double inv_a2_prev, const double* vp_table,
+                             double vp_tmin, double vp_dinv, int64_t vp_npts, int vp_odd,
+                             int64_t nx, int64_t ny, int64_t nz, uint64_t noise_seed,
+                             double noise_scale, void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    int rc = check_stage("baoab f32", stage, phi, pi, phi_out, pi_out);
+    if (rc) return rc;
+    SegTable tab;
+    rc = prepared_table("baoab f32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv, vp_odd,
+                        gamma, st, tab);
+    if (rc) return rc;
+    return langevin_real_f32(stage, phi, pi, phi_out, pi_out,
+                             make_langevin<float>(dt, dx, gamma, mu, inv_a2, inv_a2_prev,
+                                                  noise_scale),
+                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st);
+}
+
+extern "C" int tip_overdamped_c32(const float* phi1, const float* phi2, float* phi1_out,
+                                  float* phi2_out, double dt, double dx, double eta_eff,
+                                  double mu, double inv_a2, const double* vp_table,
+                                  double vp_tmin, double vp_dinv, int64_t vp_npts, double zn_n,
+                                  double zn_dv, double zn_active, int64_t nx, int64_t ny,
+                                  int64_t nz, uint64_t noise_seed, double noise_scale,
+                                  void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    TIP_REQUIRE(phi1 && phi2 && phi1_out && phi2_out && phi1 != phi1_out && phi2 != phi2_out,
+                "overdamped c32: NULL or aliased field");
+    SegTable tab;
+    int rc = prepared_table("overdamped c32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv,
+                            0, eta_eff, st, tab);
+    if (rc) return rc;
+    return langevin_complex_f32(LV_OVERDAMPED, phi1, phi2, nullptr, nullptr, phi1_out, phi2_out,
+                                nullptr, nullptr,
+                                make_langevin<float>(dt, dx, eta_eff, mu, inv_a2, inv_a2,
+                                                     noise_scale),
+                                tab, make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
+                                noise_scale != 0.0, st);
+}
+
+extern "C" int tip_baoab_c32(int stage, const float* phi1, const float* phi2, const float* pi1,
+                             const float* pi2, float* phi1_out, float* phi2_out, float* pi1_out,
+                             float* pi2_out, double dt, double dx, double gamma, double mu,
+                             double inv_a2, double inv_a2_prev, const double* vp_table,
+                             double vp_tmin, double vp_dinv, int64_t vp_npts, double zn_n,
+                             double zn_dv, double zn_active, int64_t nx, int64_t ny, int64_t nz,
+                             uint64_t noise_seed, double noise_scale, void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    int rc = check_stage("baoab c32", stage, phi1, pi1, phi1_out, pi1_out);
+    if (rc) return rc;
+    rc = check_stage("baoab c32", stage, phi2, pi2, phi2_out, pi2_out);
+    if (rc) return rc;
+    SegTable tab;
+    rc = prepared_table("baoab c32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv, 0, gamma,
+                        st, tab);
+    if (rc) return rc;
+    return langevin_complex_f32(stage, phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out, pi2_out,
+                                make_langevin<float>(dt, dx, gamma, mu, inv_a2, inv_a2_prev,
+                                                     noise_scale),
+                                tab, make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
+                                noise_scale != 0.0, st);
+}

@@ -1,7 +1,7 @@
 // tip_lattice.hip -- device-resident lattice: ping-pong field buffers, the V' table in both
 // precisions, a stream and timing events.  One RK2 step = one fused kernel launch; the
 // buffers swap roles after every step, so no copy ever runs.
-#include "step_kernels.h"
+#include "langevin_kernels.h"
 
 struct tip_lattice {
     int kind = TIP_REAL;
@@ -258,6 +258,114 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
     return TIP_OK;
 }
 
+namespace {
+
+// One overdamped / BAOAB stage on the lattice's current state.  Stages that write phi write it
+// to the other parity's phi slots; the caller swaps those (phi_swap) or the whole parity.
+int lattice_langevin(tip_lattice* L, int op, const tip_step_params* p, double inv_a2_prev,
+                     uint64_t noise_seed)
+{
+    void** in = L->buf[L->cur];
+    void** out = L->buf[L->cur ^ 1];
+    const int hash = p->noise_scale != 0.0 ? 1 : 0;
+    // BAOAB writes pi to the other parity (open / fused) or in place (close); the overdamped
+    // step leaves pi alone.
+    const bool pi_in_place = op == LV_BAOAB_CLOSE;
+    if (L->kind == TIP_REAL) {
+        void* pi_out = pi_in_place ? in[1] : out[1];
+        if (L->prec == 4) {
+            return langevin_real_f32(
+                op, (const float*)in[0], (const float*)in[1], (float*)out[0], (float*)pi_out,
+                make_langevin<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, inv_a2_prev,
+                                     p->noise_scale),
+                make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd), L->nx, L->ny,
+                L->nz, noise_seed, hash, L->stream);
+        }
+        return langevin_real_f64(
+            op, (const double*)in[0], (const double*)in[1], (double*)out[0], (double*)pi_out,
+            make_langevin<double>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, inv_a2_prev,
+                                  p->noise_scale),
+            L->table, L->tmin, L->dinv, L->npts, L->odd, L->nx, L->ny, L->nz, noise_seed, hash,
+            L->stream);
+    }
+    void* q1 = pi_in_place ? in[2] : out[2];
+    void* q2 = pi_in_place ? in[3] : out[3];
+    return langevin_complex_f32(
+        op, (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3],
+        (float*)out[0], (float*)out[1], (float*)q1, (float*)q2,
+        make_langevin<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, inv_a2_prev,
+                             p->noise_scale),
+        make_table32(L->table_ab, L->tmin, L->dinv, L->npts, 0),
+        make_zn32(L->zn_n, L->zn_dv, L->zn_active), L->nx, L->ny, L->nz, noise_seed, hash,
+        L->stream);
+}
+
+// Exchange the phi slots of the two parities (the overdamped step keeps pi where it is).
+void phi_swap(tip_lattice* L)
+{
+    const int nphi = L->kind == TIP_REAL ? 1 : 2;
+    for (int f = 0; f < nphi; ++f) {
+        void* t = L->buf[L->cur][f];
+        L->buf[L->cur][f] = L->buf[L->cur ^ 1][f];
+        L->buf[L->cur ^ 1][f] = t;
+    }
+}
+
+int langevin_checks(tip_lattice* L, const tip_step_params* p, int64_t nsteps, const char* who)
+{
+    TIP_REQUIRE(L && p, "%s: NULL", who);
+    TIP_REQUIRE(L->npts >= 2, "%s: no V' table set", who);
+    TIP_REQUIRE(nsteps >= 0, "%s: nsteps < 0", who);
+    TIP_REQUIRE(p->eta_eff > 0.0, "%s: eta_eff (gamma) must be positive", who);
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_lattice_step_overdamped(tip_lattice* L, const tip_step_params* p,
+                                           int64_t nsteps, uint64_t step0, uint64_t seed,
+                                           float* elapsed_ms)
+{
+    int rc = langevin_checks(L, p, nsteps, "tip_lattice_step_overdamped");
+    if (rc) return rc;
+    if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        rc = lattice_langevin(L, LV_OVERDAMPED, p, p->inv_a2,
+                              tip_noise_seed(step0 + (uint64_t)n, seed));
+        if (rc) return rc;
+        phi_swap(L);
+    }
+    if (elapsed_ms) {
+        TIP_HIP(hipEventRecord(L->ev1, L->stream));
+        TIP_HIP(hipEventSynchronize(L->ev1));
+        TIP_HIP(hipEventElapsedTime(elapsed_ms, L->ev0, L->ev1));
+    }
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_step_baoab(tip_lattice* L, const tip_step_params* p, int64_t nsteps,
+                                      uint64_t step0, uint64_t seed, float* elapsed_ms)
+{
+    int rc = langevin_checks(L, p, nsteps, "tip_lattice_step_baoab");
+    if (rc) return rc;
+    if (nsteps == 0) return TIP_OK;
+    if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        rc = lattice_langevin(L, n == 0 ? LV_BAOAB_OPEN : LV_BAOAB_FUSED, p, p->inv_a2,
+                              tip_noise_seed(step0 + (uint64_t)n, seed));
+        if (rc) return rc;
+        L->cur ^= 1;
+    }
+    rc = lattice_langevin(L, LV_BAOAB_CLOSE, p, p->inv_a2, 0);
+    if (rc) return rc;
+    if (elapsed_ms) {
+        TIP_HIP(hipEventRecord(L->ev1, L->stream));
+        TIP_HIP(hipEventSynchronize(L->ev1));
+        TIP_HIP(hipEventElapsedTime(elapsed_ms, L->ev0, L->ev1));
+    }
+    return TIP_OK;
+}
+
 extern "C" int tip_lattice_step_mlp(tip_lattice* L, const tip_step_params* p,
                                     const float* mlp_params, const tip_mlp_features* fm, double T,
                                     double dT_per_step, int64_t nsteps, uint64_t step0,
@@ -332,17 +440,35 @@ extern "C" int tip_lattice_vacuum_count(tip_lattice* L, double esc, unsigned lon
     return TIP_OK;
 }
 
-extern "C" int tip_lattice_winding(tip_lattice* L, int8_t* plaq_dev, unsigned long long* counts2)
+extern "C" int tip_lattice_winding(tip_lattice* L, int8_t* plaq_dev, float* wsum_dev,
+                                   unsigned long long* counts2)
 {
     TIP_REQUIRE(L && counts2, "tip_lattice_winding: NULL");
     TIP_REQUIRE(L->kind == TIP_COMPLEX, "tip_lattice_winding: needs a complex lattice");
     TIP_HIP(hipMemsetAsync(L->counter, 0, 2 * sizeof(unsigned long long), L->stream));
     void** b = L->buf[L->cur];
-    int rc = tip_winding_f32((const float*)b[0], (const float*)b[1], plaq_dev, nullptr,
+    int rc = tip_winding_f32((const float*)b[0], (const float*)b[1], plaq_dev, wsum_dev,
                              L->counter, L->nx, L->ny, L->nz, L->stream);
     if (rc) return rc;
     TIP_HIP(hipMemcpyAsync(counts2, L->counter, 2 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, L->stream));
+    TIP_HIP(hipStreamSynchronize(L->stream));
+    return TIP_OK;
+}
+
+extern "C" int tip_lattice_rho_theta(tip_lattice* L, float* rho_dev, float* theta_dev,
+                                     double* rho_minmax_host)
+{
+    TIP_REQUIRE(L, "tip_lattice_rho_theta: NULL");
+    TIP_REQUIRE(L->kind == TIP_COMPLEX && L->prec == 4,
+                "tip_lattice_rho_theta: needs a float32 complex lattice");
+    void** b = L->buf[L->cur];
+    int rc = tip_rho_theta_f32((const float*)b[0], (const float*)b[1], L->nx * L->ny * L->nz,
+                               rho_dev, theta_dev, rho_minmax_host ? L->mm : nullptr, L->stream);
+    if (rc) return rc;
+    if (rho_minmax_host)
+        TIP_HIP(hipMemcpyAsync(rho_minmax_host, L->mm, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, L->stream));
     TIP_HIP(hipStreamSynchronize(L->stream));
     return TIP_OK;
 }

@@ -89,6 +89,7 @@ SIGNATURES = {
     "tip_minmax_f32": (_int, [_vp, _i64, _vp, _vp]),
     "tip_minmax_f64": (_int, [_vp, _i64, _vp, _vp]),
     "tip_rho_max_c32": (_int, [_vp, _vp, _i64, _vp, _vp]),
+    "tip_rho_theta_f32": (_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "tip_lattice_create": (_int, [_ppv, _int, _int, _i64, _i64, _i64]),
     "tip_lattice_destroy": (_int, [_vp]),
     "tip_lattice_stream": (_int, [_vp, _ppv]),
@@ -100,8 +101,31 @@ SIGNATURES = {
     "tip_lattice_set_geometry": (_int, [_vp, _int, _int, _int, _int]),
     "tip_lattice_step": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
                                 C.POINTER(C.c_float)]),
+    "tip_lattice_step_overdamped": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
+                                           C.POINTER(C.c_float)]),
+    "tip_lattice_step_baoab": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
+                                      C.POINTER(C.c_float)]),
+    "tip_overdamped_f32": (_int, [_vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64, _int, _i64,
+                                  _i64, _i64, _u64, _d, _vp]),
+    "tip_overdamped_f64": (_int, [_vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64, _int, _i64,
+                                  _i64, _i64, _u64, _d, _vp]),
+    "tip_baoab_f32": (_int, [_int, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _vp, _d, _d,
+                             _i64, _int, _i64, _i64, _i64, _u64, _d, _vp]),
+    "tip_baoab_f64": (_int, [_int, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _d, _vp, _d, _d,
+                             _i64, _int, _i64, _i64, _i64, _u64, _d, _vp]),
+    "tip_overdamped_c32": (_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64,
+                                  _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp]),
+    "tip_overdamped_c64": (_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64,
+                                  _d, _d, _d, _i64, _i64, _i64, _u64, _d, _vp]),
+    "tip_baoab_c32": (_int, [_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d,
+                             _d, _vp, _d, _d, _i64, _d, _d, _d, _i64, _i64, _i64, _u64, _d,
+                             _vp]),
+    "tip_baoab_c64": (_int, [_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d,
+                             _d, _vp, _d, _d, _i64, _d, _d, _d, _i64, _i64, _i64, _u64, _d,
+                             _vp]),
     "tip_lattice_vacuum_count": (_int, [_vp, _d, C.POINTER(C.c_ulonglong)]),
-    "tip_lattice_winding": (_int, [_vp, _vp, C.POINTER(C.c_ulonglong)]),
+    "tip_lattice_winding": (_int, [_vp, _vp, _vp, C.POINTER(C.c_ulonglong)]),
+    "tip_lattice_rho_theta": (_int, [_vp, _vp, _vp, C.POINTER(C.c_double)]),
     "tip_lattice_minmax": (_int, [_vp, C.POINTER(C.c_double)]),
     "tip_ensemble_create": (_int, [_ppv, _int, _vp, _vp]),
     "tip_ensemble_destroy": (_int, [_vp]),

@@ -13,8 +13,14 @@ simulation/latticeSimComplex_numba.py:4237-4398):
     :4262-4287).  The field range comes from a device min/max reduction, not a host scan.
     With --surrogate the table is not used: V'(rho, T) comes from the trained network at the
     current T (complex field);
-  * noise amplitude sqrt(2 eta_eff T dt inv_a3 / (mu^2 dx_phys^3)) (:3714-3718; complex :4332-4336
-    without inv_a3) with the hash noise stream (noise_seed = step);
+  * integrator (--integrator): rk2 (rk2_step_table), baoab (baoab_step_table) or overdamped
+    (overdamped_euler_step_table); "auto" follows the reference: rk2 for the real field, for
+    the complex field overdamped when eta_phys > 100 (latticeSimComplex_numba.py:654-666) and
+    baoab otherwise (its default, :305); the surrogate path is rk2;
+  * noise amplitude: rk2 sqrt(2 eta_eff T dt inv_a3 / (mu^2 dx_phys^3)) (:3714-3718; complex
+    :4332-4336 without inv_a3), overdamped sqrt(2 T dt inv_a3 / (eta_eff mu^2 dx_phys^3))
+    (:3678; complex :4293), baoab sqrt(T inv_a3 / (mu^2 dx_phys^3)) (:3697; complex :4313),
+    with the hash noise stream (noise_seed = step);
   * observables every --report steps (true-vacuum fraction |phi| > esc, string count for the
     complex field) and a NaN/Inf/|phi| > 1e6 * esc guard (:3882-3944);
   * npz snapshots every --steps with the reference's keys (:3996-4018; complex :4632-4654).
@@ -65,6 +71,7 @@ class RunConfig:
     counterterm: bool = False  # lattice counterterm on the dJ splines (:458-490, :758-795)
     table_source: str = "spline"  # "spline": the reference's V' (dJ spline, no CW);
                                   # "exact": float64 series dJ + Coleman-Weinberg (vprime_full)
+    integrator: str = "auto"  # "rk2", "baoab", "overdamped" or "auto" (the reference's choice)
     precision: str = "f32"
     seed: int = 0
     table_size: int = 16384
@@ -89,6 +96,15 @@ class RunConfig:
             self.eta_phys = 1000.0 if cplx else self.T0  # complex :641, real eta_phys = T0 :646
         if self.potential_type is None:
             self.potential_type = "V_p" if cplx else "V_correct"
+        if self.integrator not in ("auto", "rk2", "baoab", "overdamped"):
+            raise ValueError(f"unknown integrator {self.integrator!r}")
+
+    def resolved_integrator(self) -> str:
+        if self.integrator != "auto":
+            return self.integrator
+        if self.field != "complex" or self.surrogate:
+            return "rk2"
+        return "overdamped" if self.eta_phys > 100.0 else "baoab"
 
 
 def model_for(cfg: RunConfig) -> P.ModelParams:
@@ -133,7 +149,10 @@ class Run:
         self.rebuilds = 0
         self.zn_active = 1.0 if (cfg.zn_order > 0 and cfg.zn_turn_on_T <= 0.0) else 0.0
         self.lat.set_zn(cfg.zn_order, cfg.zn_strength, self.zn_active) if self.cplx else None
+        self.integrator = cfg.resolved_integrator()
         self.surrogate = None
+        if cfg.surrogate and self.integrator != "rk2":
+            raise ValueError("the surrogate path integrates with rk2")
         if cfg.surrogate:
             if not self.cplx:
                 raise ValueError("the inline surrogate drives the complex field")
@@ -188,7 +207,12 @@ class Run:
         self.rebuilds += 1
 
     def noise_scale(self, T, eta_eff, inv_a3) -> float:
-        return math.sqrt(2.0 * eta_eff * T * self.dt * inv_a3 / (MU ** 2 * self.cfg.dx_phys ** 3))
+        dx3 = MU ** 2 * self.cfg.dx_phys ** 3
+        if self.integrator == "overdamped":
+            return math.sqrt(2.0 * T * self.dt * inv_a3 / (eta_eff * dx3))
+        if self.integrator == "baoab":
+            return math.sqrt(T * inv_a3 / dx3)
+        return math.sqrt(2.0 * eta_eff * T * self.dt * inv_a3 / dx3)
 
     # -- observables ---------------------------------------------------------------------
     def guard(self, n: int, T: float) -> None:
@@ -227,20 +251,17 @@ class Run:
         self.saves += 1
         d = dict(step=n, time=n * self.dt, temperature=T)
         if self.cplx:
-            p1, p2 = self.lat.download("phi1"), self.lat.download("phi2")
-            rho = np.sqrt(p1.astype(np.float64) ** 2 + p2.astype(np.float64) ** 2)
-            from .kernels_complex import compute_winding_number
-            w, nstr, _, _ = compute_winding_number(p1, p2)
-            d.update(phi1=p1, phi2=p2, rho=rho.astype(np.float32),
-                     theta=np.arctan2(p2.astype(np.float64), p1.astype(np.float64)).astype(np.float32),
-                     winding=w, rho_min=float(rho.min()), rho_max=float(rho.max()),
+            sp = self.lat.save_point()  # rho, theta, winding buffer and rho range on the device
+            d.update(phi1=self.lat.download("phi1"), phi2=self.lat.download("phi2"),
+                     rho=sp["rho"], theta=sp["theta"], winding=sp["winding"],
+                     rho_min=sp["rho_min"], rho_max=sp["rho_max"],
                      zn_active=float(self.zn_active))
-            if ckpt:
+            if ckpt and self.integrator != "overdamped":
                 d.update(pi1=self.lat.download("pi1"), pi2=self.lat.download("pi2"))
         else:
             phi = self.lat.download("phi")
             d.update(phi=phi, phi_min=float(phi.min()), phi_max=float(phi.max()))
-            if ckpt:
+            if ckpt and self.integrator != "overdamped":
                 d["pi"] = self.lat.download("pi")
         if self.hubble is not None:
             d.update(scale_factor=self.hubble.a)
@@ -285,7 +306,13 @@ class Run:
             else:
                 k = 1
                 self.maybe_rebuild(T)
-                self.lat.step(1, **kw)
+                if self.integrator == "baoab":
+                    kw["gamma"] = kw.pop("eta_eff")
+                    self.lat.step_baoab(1, **kw)
+                elif self.integrator == "overdamped":
+                    self.lat.step_overdamped(1, **kw)
+                else:
+                    self.lat.step(1, **kw)
             n += k
             T_now = T  # temperature of the last step taken
             if cfg.report and n % cfg.report == 0:

@@ -36,7 +36,8 @@ def surrogate_vs_table(surrogate, n: int = 64, steps: int = 4000, T0: float = 73
     for method in methods:
         F, S, el, rebuilds = [], [], 0.0, 0
         for sd in seeds:
-            cfg = RunConfig(field="complex", Nx=n, Ny=n, Nz=n, Nt=steps, T0=T0, cool_to=T1,
+            cfg = RunConfig(field="complex", integrator="rk2", Nx=n, Ny=n, Nz=n, Nt=steps,
+                            T0=T0, cool_to=T1,
                             cool_steps=cool_steps, seed=sd, report=report, esc=esc, quiet=True,
                             surrogate=surrogate if method == "surrogate" else None,
                             table_source="exact" if method == "table_exact" else "spline",

@@ -5,6 +5,10 @@ functions; the work runs in libtip's gfx950 kernels.
 
   rk2_step_table(phi, pi, dt, dx, eta_eff, mu, noise, phi_tmp, pi_tmp, inv_a2,
                  vp_table, vp_tmin, vp_dinv, vp_npts)           :1875-1966
+  overdamped_euler_step_table(phi, dt, dx, eta_eff, mu, inv_a2,
+                 vp_table, vp_tmin, vp_dinv, vp_npts, noise_scale, seed)   :2115-2192
+  baoab_step_table(phi, pi, dt, dx, gamma, mu, inv_a2,
+                 vp_table, vp_tmin, vp_dinv, vp_npts, noise_scale, seed)   :1974-2112
   generate_noise_field(noise, scale, seed)                        :1348-1361
   _hash_rng_pair(seed) -> (u1, u2)                                :2322-2336
 
@@ -132,6 +136,84 @@ def rk2_step_table(phi, pi, dt, dx, eta_eff, mu, noise, phi_tmp, pi_tmp, inv_a2,
     if host:
         d_phi.download(phi)
         d_pi.download(pi)
+
+
+BAOAB_OPEN, BAOAB_FUSED, BAOAB_CLOSE = 1, 2, 3  # TIP_BAOAB_* of include/tip.h
+
+
+def _field_set(names, arrays, dtype):
+    """Device buffers for a call: the arrays themselves when on the device, otherwise staging
+    buffers holding an upload of each (host arrays).  Returns (devices, host?)."""
+    host = not _is_dev(arrays[0])
+    if host and any(_is_dev(a) for a in arrays):
+        raise TypeError("mix of host and device arrays")
+    if not host:
+        return list(arrays), False
+    devs = []
+    for nm, a in zip(names, arrays):
+        d = _staging.get(nm, a.shape, dtype)
+        d.upload(a)
+        devs.append(d)
+    return devs, True
+
+
+def _write_back(host, dst, src):
+    """dst <- src after a call: a download into host arrays, a device copy otherwise."""
+    for a, d in zip(dst, src):
+        if host:
+            d.download(a)
+        else:
+            a.copy_from(d)
+
+
+def overdamped_euler_step_table(phi, dt, dx, eta_eff, mu, inv_a2, vp_table, vp_tmin, vp_dinv,
+                                vp_npts, noise_scale, seed, *, odd: bool = False, stream=None):
+    """One overdamped Euler-Maruyama step of the real field, in place on phi.
+
+    Reference: overdamped_euler_step_table, simulation/latticeSimeRescale_numba.py:2115-2192,
+    evaluated on one consistent field (the reference's in-place loop lets part of each
+    Laplacian see advanced neighbours).  seed is the reference's per-call seed (the step
+    index); the noise is the hash stream of noise_seed = seed.
+    """
+    _check_field("phi", phi)
+    shape, dtype = tuple(phi.shape), np.dtype(phi.dtype)
+    if dtype not in (np.float32, np.float64):
+        raise TypeError(f"phi: unsupported dtype {dtype}")
+    table = _table_dev(vp_table, int(vp_npts))
+    (d_phi,), host = _field_set(["od_phi"], [phi], dtype)
+    out = _staging.get("od_out", shape, dtype)
+    name = "tip_overdamped_f64" if dtype == np.float64 else "tip_overdamped_f32"
+    _lib.call(name, d_phi.ptr, out.ptr, float(dt), float(dx), float(eta_eff), float(mu),
+              float(inv_a2), table.ptr, float(vp_tmin), float(vp_dinv), int(vp_npts),
+              int(bool(odd)), *shape, int(seed) & 0xFFFFFFFFFFFFFFFF, float(noise_scale), stream)
+    _write_back(host, [phi], [out])
+
+
+def baoab_step_table(phi, pi, dt, dx, gamma, mu, inv_a2, vp_table, vp_tmin, vp_dinv, vp_npts,
+                     noise_scale, seed, *, odd: bool = False, stream=None):
+    """One BAOAB Langevin step of the real field, in place on (phi, pi).
+
+    Reference: baoab_step_table, simulation/latticeSimeRescale_numba.py:1974-2112 (two lattice
+    passes: B A O A, then the closing B on the advanced field), with every force evaluated on
+    one consistent field.  noise_scale is the reference's BAOAB amplitude sqrt(T / (mu^2
+    dx^3)); c2 = sqrt(1 - exp(-2 gamma dt)) noise_scale.
+    """
+    _check_field("phi", phi)
+    shape, dtype = tuple(phi.shape), np.dtype(phi.dtype)
+    if dtype not in (np.float32, np.float64):
+        raise TypeError(f"phi: unsupported dtype {dtype}")
+    _check_field("pi", pi, shape, dtype)
+    table = _table_dev(vp_table, int(vp_npts))
+    (d_phi, d_pi), host = _field_set(["ba_phi", "ba_pi"], [phi, pi], dtype)
+    po = _staging.get("ba_phi_out", shape, dtype)
+    qo = _staging.get("ba_pi_out", shape, dtype)
+    name = "tip_baoab_f64" if dtype == np.float64 else "tip_baoab_f32"
+    common = (float(dt), float(dx), float(gamma), float(mu), float(inv_a2), float(inv_a2),
+              table.ptr, float(vp_tmin), float(vp_dinv), int(vp_npts), int(bool(odd)), *shape)
+    _lib.call(name, BAOAB_OPEN, d_phi.ptr, d_pi.ptr, po.ptr, qo.ptr, *common,
+              int(seed) & 0xFFFFFFFFFFFFFFFF, float(noise_scale), stream)
+    _lib.call(name, BAOAB_CLOSE, po.ptr, qo.ptr, None, qo.ptr, *common, 0, 0.0, stream)
+    _write_back(host, [phi, pi], [po, qo])
 
 
 def generate_noise_field(noise, scale, seed, *, mult: int = NOISE_MULT_PASS2, noise2=None,

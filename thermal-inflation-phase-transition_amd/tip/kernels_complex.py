@@ -5,6 +5,10 @@ optional Z_N term).
   rk2_step_table(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, noise2,
                  phi1_tmp, phi2_tmp, pi1_tmp, pi2_tmp, inv_a2,
                  vp_table, vp_tmin, vp_dinv, vp_npts)                :2067-2215
+  overdamped_euler_step_table(phi1, phi2, dt, dx, eta_eff, mu, inv_a2,
+                 vp_table, vp_tmin, vp_dinv, vp_npts, noise_scale, seed)  :2422-2517
+  baoab_step_table(phi1, phi2, pi1, pi2, dt, dx, gamma, mu, inv_a2,
+                 vp_table, vp_tmin, vp_dinv, vp_npts, noise_scale, seed)  :2223-2419
   compute_winding_number(phi1, phi2, winding_out)                    :2683-2744
 
 The Z_N parameters, a module global in the reference (_ZN_PARAMS = [N, delta_V, active],
@@ -16,7 +20,8 @@ import numpy as np
 
 from . import _lib
 from .device import Counter, DeviceArray
-from .kernels import _check_field, _is_dev, _staging, _table_dev
+from .kernels import (BAOAB_CLOSE, BAOAB_OPEN, _check_field, _field_set, _is_dev, _staging,
+                      _table_dev, _write_back)
 
 # [N (order), delta_V (strength), active flag] -- same meaning as the reference's global
 ZN_PARAMS = np.array([0.0, 0.0, 0.0], dtype=np.float64)
@@ -71,6 +76,62 @@ def rk2_step_table(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, noise2, ph
     if host:
         for d, a in zip(devs[:4], fields[:4]):
             d.download(a)
+
+
+def _complex_fields(names, arrays):
+    _check_field(names[0], arrays[0])
+    shape, dtype = tuple(arrays[0].shape), np.dtype(arrays[0].dtype)
+    if dtype not in (np.float32, np.float64):
+        raise TypeError(f"{names[0]}: unsupported dtype {dtype}")
+    for nm, a in zip(names[1:], arrays[1:]):
+        _check_field(nm, a, shape, dtype)
+    return shape, dtype
+
+
+def overdamped_euler_step_table(phi1, phi2, dt, dx, eta_eff, mu, inv_a2, vp_table, vp_tmin,
+                                vp_dinv, vp_npts, noise_scale, seed, *, zn=None, stream=None):
+    """One overdamped Euler-Maruyama step of the complex field, in place on (phi1, phi2).
+
+    Reference: overdamped_euler_step_table, simulation/latticeSimComplex_numba.py:2422-2517,
+    evaluated on one consistent field; noise (cos, sin) of the hash draw of noise_seed = seed.
+    """
+    shape, dtype = _complex_fields(["phi1", "phi2"], [phi1, phi2])
+    zn = ZN_PARAMS if zn is None else np.asarray(zn, dtype=np.float64)
+    table = _table_dev(vp_table, int(vp_npts))
+    devs, host = _field_set(["cod_phi1", "cod_phi2"], [phi1, phi2], dtype)
+    outs = [_staging.get(nm, shape, dtype) for nm in ("cod_out1", "cod_out2")]
+    name = "tip_overdamped_c64" if dtype == np.float64 else "tip_overdamped_c32"
+    _lib.call(name, devs[0].ptr, devs[1].ptr, outs[0].ptr, outs[1].ptr, float(dt), float(dx),
+              float(eta_eff), float(mu), float(inv_a2), table.ptr, float(vp_tmin),
+              float(vp_dinv), int(vp_npts), float(zn[0]), float(zn[1]), float(zn[2]), *shape,
+              int(seed) & 0xFFFFFFFFFFFFFFFF, float(noise_scale), stream)
+    _write_back(host, [phi1, phi2], outs)
+
+
+def baoab_step_table(phi1, phi2, pi1, pi2, dt, dx, gamma, mu, inv_a2, vp_table, vp_tmin,
+                     vp_dinv, vp_npts, noise_scale, seed, *, zn=None, stream=None):
+    """One BAOAB step of the complex field, in place on (phi1, phi2, pi1, pi2).
+
+    Reference: baoab_step_table, simulation/latticeSimComplex_numba.py:2223-2419, with every
+    force evaluated on one consistent field; noise_scale is the reference's sqrt(T / (mu^2
+    dx^3)) (:4313).
+    """
+    names = ["cba_phi1", "cba_phi2", "cba_pi1", "cba_pi2"]
+    fields = [phi1, phi2, pi1, pi2]
+    shape, dtype = _complex_fields(["phi1", "phi2", "pi1", "pi2"], fields)
+    zn = ZN_PARAMS if zn is None else np.asarray(zn, dtype=np.float64)
+    table = _table_dev(vp_table, int(vp_npts))
+    devs, host = _field_set(names, fields, dtype)
+    outs = [_staging.get(nm + "_out", shape, dtype) for nm in names]
+    name = "tip_baoab_c64" if dtype == np.float64 else "tip_baoab_c32"
+    tail = (float(dt), float(dx), float(gamma), float(mu), float(inv_a2), float(inv_a2),
+            table.ptr, float(vp_tmin), float(vp_dinv), int(vp_npts), float(zn[0]),
+            float(zn[1]), float(zn[2]), *shape)
+    _lib.call(name, BAOAB_OPEN, *(d.ptr for d in devs), *(o.ptr for o in outs), *tail,
+              int(seed) & 0xFFFFFFFFFFFFFFFF, float(noise_scale), stream)
+    _lib.call(name, BAOAB_CLOSE, outs[0].ptr, outs[1].ptr, outs[2].ptr, outs[3].ptr, None, None,
+              outs[2].ptr, outs[3].ptr, *tail, 0, 0.0, stream)
+    _write_back(host, fields, outs)
 
 
 def compute_winding_number(phi1, phi2, winding_out=None, *, plaquettes: bool = False):

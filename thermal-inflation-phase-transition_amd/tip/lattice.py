@@ -90,6 +90,31 @@ class Lattice:
         self.step_count = s0 + int(nsteps)
         return float(ms.value) if timed else None
 
+    def _langevin(self, fn, nsteps, dt, dx, eta, mu, inv_a2, noise_scale, seed, step0, timed):
+        p = StepParams(dt, dx, eta, mu, inv_a2, noise_scale)
+        s0 = self.step_count if step0 is None else int(step0)
+        ms = C.c_float(0.0)
+        _lib.call(fn, self._h, C.byref(p), int(nsteps), s0 & 0xFFFFFFFFFFFFFFFF,
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(ms) if timed else None)
+        self.step_count = s0 + int(nsteps)
+        return float(ms.value) if timed else None
+
+    def step_overdamped(self, nsteps: int, *, dt: float, dx: float, eta_eff: float, mu: float,
+                        inv_a2: float = 1.0, noise_scale: float = 0.0, seed: int = 0,
+                        step0: Optional[int] = None, timed: bool = False) -> Optional[float]:
+        """Advance nsteps of overdamped Euler-Maruyama (overdamped_euler_step_table);
+        noise_scale = sqrt(2 T dt / (eta_eff mu^2 dx^3)); pi is left untouched."""
+        return self._langevin("tip_lattice_step_overdamped", nsteps, dt, dx, eta_eff, mu,
+                              inv_a2, noise_scale, seed, step0, timed)
+
+    def step_baoab(self, nsteps: int, *, dt: float, dx: float, gamma: float, mu: float,
+                   inv_a2: float = 1.0, noise_scale: float = 0.0, seed: int = 0,
+                   step0: Optional[int] = None, timed: bool = False) -> Optional[float]:
+        """Advance nsteps of BAOAB (baoab_step_table) in nsteps + 1 lattice passes;
+        noise_scale = sqrt(T / (mu^2 dx^3))."""
+        return self._langevin("tip_lattice_step_baoab", nsteps, dt, dx, gamma, mu, inv_a2,
+                              noise_scale, seed, step0, timed)
+
     def step_mlp(self, nsteps: int, surrogate, *, T_start: float, dT_per_step: float = 0.0,
                  dt: float, dx: float, eta_eff: float, mu: float, inv_a2: float = 1.0,
                  noise_scale: float = 0.0, seed: int = 0, step0: Optional[int] = None,
@@ -119,11 +144,39 @@ class Lattice:
     def vacuum_fraction(self, esc: float = 1e4) -> float:
         return self.vacuum_count(esc) / float(np.prod(self.shape))
 
-    def winding_counts(self, plaq_ptr: Optional[int] = None):
-        """(string sites, nonzero plaquettes) of the current complex field."""
+    def winding_counts(self, plaq_ptr: Optional[int] = None, wsum_ptr: Optional[int] = None):
+        """(string sites, nonzero plaquettes) of the current complex field; optionally the
+        int8 [3][N] plaquette windings at plaq_ptr and the float32 per-site winding density
+        (the reference's winding buffer) at wsum_ptr, both device pointers."""
         c = (C.c_ulonglong * 2)()
-        _lib.call("tip_lattice_winding", self._h, plaq_ptr, c)
+        _lib.call("tip_lattice_winding", self._h, plaq_ptr, wsum_ptr, c)
         return int(c[0]), int(c[1])
+
+    def rho_theta(self, rho_ptr: Optional[int] = None, theta_ptr: Optional[int] = None):
+        """Fill rho = |phi| and theta = arg(phi) (float32, formed in float64) at the device
+        pointers given; returns (rho_min, rho_max) of the float64 rho."""
+        mm = (C.c_double * 2)()
+        _lib.call("tip_lattice_rho_theta", self._h, rho_ptr, theta_ptr, mm)
+        return float(mm[0]), float(mm[1])
+
+    def save_point(self) -> dict:
+        """The complex driver's save-point quantities computed on the device
+        (latticeSimComplex_numba.py:4625-4646): rho, theta, the winding buffer, rho_min,
+        rho_max and the string count; only the finished float32 arrays cross PCIe."""
+        if not self.complex:
+            raise ValueError("save_point needs a complex lattice")
+        from .device import DeviceArray
+        rho = DeviceArray(self.shape, np.float32)
+        theta = DeviceArray(self.shape, np.float32)
+        wind = DeviceArray(self.shape, np.float32)
+        try:
+            lo, hi = self.rho_theta(rho.ptr, theta.ptr)
+            nstr, _ = self.winding_counts(None, wind.ptr)
+            return dict(rho=rho.download(), theta=theta.download(), winding=wind.download(),
+                        rho_min=lo, rho_max=hi, n_strings=nstr)
+        finally:
+            for a in (rho, theta, wind):
+                a.free()
 
     def minmax(self):
         out = (C.c_double * 2)()
