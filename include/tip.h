@@ -250,6 +250,31 @@ int tip_winding_f64(const double *phi1, const double *phi2, int8_t *plaq, float 
                     unsigned long long *counts, int64_t nx, int64_t ny, int64_t nz,
                     void *stream);
 
+/* Connected-component labelling of {|v| > threshold} (the comparison in v's precision), as
+ * scipy.ndimage.label with its default 6-connectivity structure numbers them for the bubble
+ * (|phi| > esc) and string (|W| > 0.5) analysis of postprocess/revisualize_snapshots.py:
+ * 171-213 (_identify_bubbles) and :360-392 (_identify_strings).  labels (device int32 [N]):
+ * 0 = background, components 1..n in raster order of their first site (scipy's numbering);
+ * *n_components (host) = n.  periodic != 0 also joins across the lattice faces (the reference's
+ * analysis does not).  Synchronous on the stream; uses a per-device scratch of 4 N bytes. */
+int tip_label_f32(const float *v, double threshold, int64_t nx, int64_t ny, int64_t nz,
+                  int periodic, int32_t *labels, int64_t *n_components, void *stream);
+int tip_label_f64(const double *v, double threshold, int64_t nx, int64_t ny, int64_t nz,
+                  int periodic, int32_t *labels, int64_t *n_components, void *stream);
+/* Per-component sums over the labelled sites (device outputs, n_components entries each,
+ * sum_xyz 3 per component): volume, sum of x, y, z (site coordinates), sum of x^2 + y^2 + z^2
+ * (exact integers), sum of v (float64) and max |v| (float64 bit patterns).  From these the
+ * host forms the reference's centroid, r_eff = (3V/4pi)^(1/3), r_rms, sign of the mean and
+ * peak (revisualize_snapshots.py:189-205, :372-388). */
+int tip_label_stats_f32(const float *v, const int32_t *labels, int64_t n_components, int64_t nx,
+                        int64_t ny, int64_t nz, unsigned long long *volume,
+                        unsigned long long *sum_xyz, unsigned long long *sum_sq, double *sum_v,
+                        unsigned long long *peak_bits, void *stream);
+int tip_label_stats_f64(const double *v, const int32_t *labels, int64_t n_components,
+                        int64_t nx, int64_t ny, int64_t nz, unsigned long long *volume,
+                        unsigned long long *sum_xyz, unsigned long long *sum_sq, double *sum_v,
+                        unsigned long long *peak_bits, void *stream);
+
 /* True-vacuum count: += #sites with |phi| > esc (real) or phi1^2 + phi2^2 > esc^2 (complex).
  * Replaces the off-line mean(|phi| > esc) of postprocess/compute_lattice_Tc1.py:24 and the
  * falseVacuumFraction reduction of cosmolattice_ext/models/thermal_inflation.h:331-358. */

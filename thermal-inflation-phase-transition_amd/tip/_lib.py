@@ -123,6 +123,12 @@ SIGNATURES = {
     "tip_baoab_c64": (_int, [_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d,
                              _d, _vp, _d, _d, _i64, _d, _d, _d, _i64, _i64, _i64, _u64, _d,
                              _vp]),
+    "tip_label_f32": (_int, [_vp, _d, _i64, _i64, _i64, _int, _vp, C.POINTER(C.c_int64), _vp]),
+    "tip_label_f64": (_int, [_vp, _d, _i64, _i64, _i64, _int, _vp, C.POINTER(C.c_int64), _vp]),
+    "tip_label_stats_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
+                                   _vp]),
+    "tip_label_stats_f64": (_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
+                                   _vp]),
     "tip_lattice_vacuum_count": (_int, [_vp, _d, C.POINTER(C.c_ulonglong)]),
     "tip_lattice_winding": (_int, [_vp, _vp, _vp, C.POINTER(C.c_ulonglong)]),
     "tip_lattice_rho_theta": (_int, [_vp, _vp, _vp, C.POINTER(C.c_double)]),

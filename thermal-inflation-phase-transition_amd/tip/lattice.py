@@ -183,6 +183,38 @@ class Lattice:
         _lib.call("tip_lattice_minmax", self._h, out)
         return float(out[0]), float(out[1])
 
+    def stream(self) -> int:
+        """The lattice's HIP stream (hipStream_t as an integer)."""
+        p = C.c_void_p()
+        _lib.call("tip_lattice_stream", self._h, C.byref(p))
+        return int(p.value or 0)
+
+    def bubbles(self, esc: float = 1e4, dx_phys: Optional[float] = None,
+                periodic: bool = False):
+        """Connected regions of |phi| > esc of the current real field with the reference's
+        per-bubble measures (tip.analysis.identify_bubbles), labelled on the device."""
+        if self.complex:
+            raise TypeError("bubbles: needs a real lattice")
+        from .analysis import identify_bubbles
+        from .device import DeviceArray
+        v = DeviceArray(self.shape, self.dtype, ptr=self.field_ptr("phi"))
+        _lib.call("tip_stream_synchronize", self.stream())  # steps in flight on the lattice
+        return identify_bubbles(v, esc, dx_phys, periodic)
+
+    def strings(self, threshold: float = 0.5, periodic: bool = False):
+        """(labels, strings) of the current complex field: the winding buffer and its
+        components (tip.analysis.identify_strings), both computed on the device."""
+        if not self.complex:
+            raise TypeError("strings: needs a complex lattice")
+        from .analysis import identify_strings
+        from .device import DeviceArray
+        w = DeviceArray(self.shape, np.float32)
+        try:
+            self.winding_counts(None, w.ptr)
+            return identify_strings(w, threshold, periodic)
+        finally:
+            w.free()
+
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             _lib.load().tip_lattice_destroy(self._h)
