@@ -250,7 +250,93 @@ def bench_extras(steps: int = 200) -> dict:
                            "grid_mode_hbm_gbps_algorithmic": 32.0 * m ** 3 / (us_g * 1e-6) / 1e9}
     lat.close()
     log(f"complex 512^3 with the inline MLP: {us:.1f} us/step; per-step grid: {us_g:.1f} us/step")
+    out.update(bench_other_integrators(steps))
+    out["labels256"] = bench_labels()
     return out
+
+
+def bench_other_integrators(steps: int = 200) -> dict:
+    """The reference's BAOAB and overdamped integrators at 256^3 (real and complex float32) on
+    a resident lattice: BAOAB k steps = k + 1 passes (16 / 32 B per site and pass), the
+    overdamped step reads and writes phi only (8 / 16 B per site)."""
+    from tip import potential as P
+    from tip.lattice import Lattice
+
+    out = {}
+    n = 256
+    shape = (n, n, n)
+    table, tmin, dinv, prm = real_workload(n)
+    rng = np.random.default_rng(5)
+    phi0 = (0.01 * rng.normal(size=shape)).astype(np.float32)
+    pc = P.ModelParams.complex()
+    tc, tcmin, tcdinv = P.build_vprime_table(6770.0, 0.0, 1e5, 16384, pc)
+    th = 2 * np.pi * rng.random(shape)
+    r = 50.0 * np.abs(rng.normal(size=shape))
+    for field in ("real", "complex"):
+        for kind in ("baoab", "overdamped"):
+            cplx = field == "complex"
+            lat = Lattice(shape, np.float32, complex_field=cplx)
+            if cplx:
+                lat.upload("phi1", (r * np.cos(th)).astype(np.float32))
+                lat.upload("phi2", (r * np.sin(th)).astype(np.float32))
+                lat.upload("pi1", np.zeros(shape, np.float32))
+                lat.upload("pi2", np.zeros(shape, np.float32))
+                lat.set_table(tc, tcmin, tcdinv)
+            else:
+                lat.upload("phi", phi0)
+                lat.upload("pi", np.zeros(shape, np.float32))
+                lat.set_table(table, tmin, dinv, odd=True)
+            kw = dict(dt=prm["dt"], dx=prm["dx"], mu=prm["mu"], noise_scale=30.0, seed=1)
+            if kind == "baoab":
+                run = lambda k: lat.step_baoab(k, gamma=prm["eta_eff"], timed=True, **kw)
+            else:
+                run = lambda k: lat.step_overdamped(k, eta_eff=prm["eta_eff"], timed=True, **kw)
+            run(10)
+            us = 1e3 * run(steps) / steps
+            lat.close()
+            b = (16.0 if kind == "baoab" else 8.0) * (2 if cplx else 1)
+            passes = (steps + 1) / steps if kind == "baoab" else 1.0
+            out[f"{kind}_{field}256"] = {
+                "us_per_step": us, "site_updates_per_s": n ** 3 / (us * 1e-6),
+                "bytes_per_site_pass": b, "passes_per_step": passes,
+                "hbm_gbps_algorithmic": b * passes * n ** 3 / (us * 1e-6) / 1e9}
+            log(f"{kind} {field} 256^3: {us:.1f} us/step")
+    return out
+
+
+def bench_labels() -> dict:
+    """Component labelling of |phi| > esc at 256^3 (a smoothed field near the percolation
+    threshold): the device labels and measures against scipy.ndimage.label plus the
+    reference's numpy measures on the host cores."""
+    from scipy import ndimage
+
+    from tip import analysis as A
+    from tip.device import DeviceArray
+
+    n = 256
+    rng = np.random.default_rng(9)
+    f = ndimage.gaussian_filter(rng.normal(size=(n, n, n)), 1.5, mode="wrap")
+    phi = (f / f.std() * 1e4).astype(np.float32)
+    d = DeviceArray.from_numpy(phi)
+    A.identify_bubbles(d, 5e3)  # warm
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        comp = A.Components(d, 5e3, stats=False)
+        comp.close()
+    lab_ms = 1e3 * (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bub = A.identify_bubbles(d, 5e3)
+    all_ms = 1e3 * (time.perf_counter() - t0) / reps
+    d.free()
+    t0 = time.perf_counter()
+    lab, nc = ndimage.label((phi > 5e3) | (phi < -5e3))
+    sp_ms = 1e3 * (time.perf_counter() - t0)
+    log(f"labels 256^3: {len(bub)} components, device labelling {lab_ms:.2f} ms, "
+        f"with measures {all_ms:.2f} ms; scipy.ndimage.label {sp_ms:.1f} ms")
+    return {"components": len(bub), "label_ms_wall": lab_ms, "label_and_measures_ms_wall": all_ms,
+            "scipy_label_ms": sp_ms, "scipy_components": int(nc)}
 
 
 def main() -> int:

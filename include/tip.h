@@ -275,6 +275,16 @@ int tip_label_stats_f64(const double *v, const int32_t *labels, int64_t n_compon
                         unsigned long long *sum_xyz, unsigned long long *sum_sq, double *sum_v,
                         unsigned long long *peak_bits, void *stream);
 
+/* The CosmoLattice port's numba-compatible initial field (applyNumbaInitialConditions,
+ * cosmolattice_ext/models/thermal_inflation.h:174-222): fld0[s] = amp * z(s ^ kIcSeed),
+ * fld1[s] (nullable) = amp * z(s ^ 0x9E3779B97F4A7C15 ^ kIcSeed), kIcSeed = 0x4E554D42412D4943,
+ * z = the port's hashGaussian of the splitmix64 finaliser, s = (ix ny + iy) nz + iz; computed
+ * in float64 and rounded to the output type. */
+int tip_cl_hash_ics_f32(float *fld0, float *fld1, int64_t nx, int64_t ny, int64_t nz,
+                        double amp, void *stream);
+int tip_cl_hash_ics_f64(double *fld0, double *fld1, int64_t nx, int64_t ny, int64_t nz,
+                        double amp, void *stream);
+
 /* True-vacuum count: += #sites with |phi| > esc (real) or phi1^2 + phi2^2 > esc^2 (complex).
  * Replaces the off-line mean(|phi| > esc) of postprocess/compute_lattice_Tc1.py:24 and the
  * falseVacuumFraction reduction of cosmolattice_ext/models/thermal_inflation.h:331-358. */

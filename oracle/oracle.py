@@ -80,6 +80,7 @@ def lib():
         L.orc_langevin_c64.argtypes = [
             _int, _f64p, _f64p, _nf64, _nf64, _nf64, _nf64, _nf64, _nf64, _d, _d, _d, _d, _d, _d,
             _f64p, _d, _d, _i64, _d, _d, _d, _u64, _d, _i64, _i64, _i64]
+        L.orc_cl_hash_ics.argtypes = [_f64p, _nf64, _i64, _d]
         L.orc_winding.argtypes = [_f64p, _f64p, _nf32, _ni8, _i64, _i64, _i64]
         L.orc_string_count.argtypes = [_f32p, _i64]
         L.orc_string_count.restype = _i64
@@ -307,3 +308,13 @@ def vacuum_count(phi, esc, phi2=None):
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def cl_hash_ics(shape, amp, two=False):
+    """The CosmoLattice port's hash initial conditions (tip_oracle.c orc_cl_hash_ics,
+    cosmolattice_ext/models/thermal_inflation.h:174-222); float64 (f0, f1 or None)."""
+    n = int(np.prod(shape))
+    f0 = np.empty(n)
+    f1 = np.empty(n) if two else None
+    lib().orc_cl_hash_ics(f0, f1, n, float(amp))
+    return f0.reshape(shape), (None if f1 is None else f1.reshape(shape))

@@ -641,3 +641,39 @@ ORC_API void orc_langevin_c64(int op, const double *phi1, const double *phi2, co
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * CosmoLattice port's numba-compatible initial conditions,
+ * cosmolattice_ext/models/thermal_inflation.h:174-222 (globalSiteIndex, hashMix64,
+ * hashGaussian, applyNumbaInitialConditions): component c of site s is
+ * amp * hashGaussian(s ^ kIcSeed) (c = 0) or amp * hashGaussian(s ^ golden ^ kIcSeed) (c = 1).
+ * ------------------------------------------------------------------------------------------ */
+static inline uint64_t cl_mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    return x;
+}
+
+static inline double cl_hash_gaussian(uint64_t seed)
+{
+    uint64_t h1 = cl_mix64(seed);
+    uint64_t h2 = cl_mix64(seed ^ 0x9E3779B97F4A7C15ULL);
+    double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+    double r = sqrt(-2.0 * log(u1 > 1e-300 ? u1 : 1e-300));
+    return r * cos(6.28318530717958647692 * u2);
+}
+
+ORC_API void orc_cl_hash_ics(double *f0, double *f1, int64_t n, double amp)
+{
+    const uint64_t ic = 0x4E554D42412D4943ULL;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < n; ++s) {
+        f0[s] = amp * cl_hash_gaussian((uint64_t)s ^ ic);
+        if (f1) f1[s] = amp * cl_hash_gaussian((uint64_t)s ^ 0x9E3779B97F4A7C15ULL ^ ic);
+    }
+}

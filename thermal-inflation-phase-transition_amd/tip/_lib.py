@@ -129,6 +129,8 @@ SIGNATURES = {
                                    _vp]),
     "tip_label_stats_f64": (_int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
                                    _vp]),
+    "tip_cl_hash_ics_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _d, _vp]),
+    "tip_cl_hash_ics_f64": (_int, [_vp, _vp, _i64, _i64, _i64, _d, _vp]),
     "tip_lattice_vacuum_count": (_int, [_vp, _d, C.POINTER(C.c_ulonglong)]),
     "tip_lattice_winding": (_int, [_vp, _vp, _vp, C.POINTER(C.c_ulonglong)]),
     "tip_lattice_rho_theta": (_int, [_vp, _vp, _vp, C.POINTER(C.c_double)]),

@@ -84,15 +84,33 @@ class Components:
     def labels_host(self) -> np.ndarray:
         return self.labels.download()
 
-    def centroid(self, c: int) -> np.ndarray:
-        v = int(self.volume[c])
-        return np.array([int(self.sum_xyz[c, a]) / v for a in range(3)], dtype=np.float64)
+    def centroids(self) -> np.ndarray:
+        """(n, 3) mean site coordinates: exact integer sums divided once (numpy's mean of the
+        integer coordinates gives the same correctly rounded value while the sums stay below
+        2^53)."""
+        if self.n == 0:
+            return np.zeros((0, 3))
+        if int(self.sum_xyz.max()) < (1 << 53):
+            return self.sum_xyz.astype(np.float64) / self.volume[:, None].astype(np.float64)
+        return np.array([[int(self.sum_xyz[c, a]) / int(self.volume[c]) for a in range(3)]
+                         for c in range(self.n)], dtype=np.float64)
 
-    def r_rms(self, c: int) -> float:
-        """sqrt(mean |r - centroid|^2) from the exact integer sums."""
-        v = int(self.volume[c])
-        num = v * int(self.sum_sq[c]) - sum(int(self.sum_xyz[c, a]) ** 2 for a in range(3))
-        return math.sqrt(num / (v * v))
+    def r_rms(self) -> np.ndarray:
+        """sqrt(mean |r - centroid|^2) per component from the exact integer numerator
+        V sum|r|^2 - |sum r|^2 (int64 where it cannot overflow, Python integers otherwise)."""
+        if self.n == 0:
+            return np.zeros(0)
+        v = self.volume.astype(np.int64)
+        if int(v.max()) * int(self.sum_sq.max()) < (1 << 62):
+            sx = self.sum_xyz.astype(np.int64)
+            num = v * self.sum_sq.astype(np.int64) - (sx * sx).sum(axis=1)
+            return np.sqrt(num.astype(np.float64) / (v.astype(np.float64) ** 2))
+        out = np.empty(self.n)
+        for c in range(self.n):
+            vc = int(v[c])
+            num = vc * int(self.sum_sq[c]) - sum(int(self.sum_xyz[c, a]) ** 2 for a in range(3))
+            out[c] = math.sqrt(num / (vc * vc))
+        return out
 
     def close(self) -> None:
         if getattr(self, "labels", None) is not None:
@@ -124,11 +142,18 @@ def identify_bubbles(phi, esc: float, dx_phys: Optional[float] = None,
     comp = Components(phi, esc, periodic)
     try:
         out = []
+        if comp.n == 0:
+            return out
+        cen, rr = comp.centroids(), comp.r_rms()
+        vols = comp.volume.tolist()
+        signs = np.where(comp.sum_v >= 0, 1, -1).tolist()
+        peaks = comp.peak.tolist()
+        rrl = rr.tolist()
         for c in range(comp.n):
-            vol = int(comp.volume[c])
-            e = dict(centroid=comp.centroid(c), volume=vol,
-                     r_eff=(3.0 * vol / (4.0 * np.pi)) ** (1.0 / 3.0), r_rms=comp.r_rms(c),
-                     sign=1 if comp.sum_v[c] >= 0 else -1, peak=float(comp.peak[c]))
+            vol = vols[c]
+            e = dict(centroid=cen[c], volume=vol,
+                     r_eff=(3.0 * vol / (4.0 * np.pi)) ** (1.0 / 3.0), r_rms=rrl[c],
+                     sign=signs[c], peak=peaks[c])
             if dx_phys is not None:
                 e["r_phys"] = e["r_eff"] * dx_phys
             out.append(e)
@@ -145,11 +170,13 @@ def identify_strings(winding, threshold: float = 0.5, periodic: bool = False):
     comp = Components(winding, threshold, periodic)
     try:
         strings = []
+        cen = comp.centroids()
+        vols = comp.volume.tolist() if comp.n else []
+        signs = np.where(comp.sum_v >= 0, 1, -1).tolist() if comp.n else []
+        peaks = comp.peak.tolist() if comp.n else []
         for c in range(comp.n):
-            strings.append(dict(loop_id=c + 1, n_voxels=int(comp.volume[c]),
-                                centroid=comp.centroid(c),
-                                winding_sign=1 if comp.sum_v[c] >= 0 else -1,
-                                max_winding=float(comp.peak[c])))
+            strings.append(dict(loop_id=c + 1, n_voxels=vols[c], centroid=cen[c],
+                                winding_sign=signs[c], max_winding=peaks[c]))
         strings.sort(key=lambda s: s["n_voxels"], reverse=True)
         return comp.labels_host(), strings
     finally:

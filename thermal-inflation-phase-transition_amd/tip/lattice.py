@@ -183,6 +183,29 @@ class Lattice:
         _lib.call("tip_lattice_minmax", self._h, out)
         return float(out[0]), float(out[1])
 
+    def save_cosmolattice(self, path: str, *, step: int, t: float, T: float, a: float = 1.0,
+                          H: float = 0.0, fStar: float = 1.0) -> None:
+        """Write the current state as a CosmoLattice FLPI / FLP2 raw snapshot (program units
+        = GeV / fStar; cubic lattices only)."""
+        from .cosmolattice import write_raw_snapshot
+        names = ["phi1", "phi2", "pi1", "pi2"] if self.complex else ["phi", "pi"]
+        write_raw_snapshot(path, {k: self.download(k) for k in names}, step=step, t=t, T=T, a=a,
+                           H=H, fStar=fStar)
+
+    def load_cosmolattice(self, path: str) -> dict:
+        """Upload a CosmoLattice raw snapshot (converted to GeV) into this lattice; returns its
+        header values."""
+        from .cosmolattice import read_raw_snapshot
+        snap = read_raw_snapshot(path, derived=False)
+        names = ["phi1", "phi2", "pi1", "pi2"] if self.complex else ["phi", "pi"]
+        if snap["n_scalars"] != (2 if self.complex else 1):
+            raise ValueError("snapshot and lattice differ in the number of scalars")
+        for k in names:
+            if k not in snap:
+                raise ValueError(f"snapshot has no {k}")
+            self.upload(k, snap[k].astype(self.dtype))
+        return {k: v for k, v in snap.items() if not isinstance(v, np.ndarray)}
+
     def stream(self) -> int:
         """The lattice's HIP stream (hipStream_t as an integer)."""
         p = C.c_void_p()
