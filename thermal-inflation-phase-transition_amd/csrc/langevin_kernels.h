@@ -138,9 +138,28 @@ __global__ __launch_bounds__(64 * BY) void k_langevin_real(const R* __restrict__
     }
 }
 
+// Complex force of the two integrators (latticeSimComplex_numba.py:2286-2327, 2369-2412,
+// 2475-2516):
+// float64 in the reference's order (complex_force); float32 with the hardware square root and
+// reciprocal of the RK2 row kernel (complex_force_f32).
+template <typename Tab>
+__device__ __forceinline__ void lv_complex_force(const Tab& tab, const ZnParams<double>& zn,
+                                                 double a, double b, double& d1, double& d2)
+{
+    complex_force(tab, zn, a, b, d1, d2);
+}
+__device__ __forceinline__ void lv_complex_force(const TableF32<false>& tab,
+                                                 const ZnParams<float>& zn, float a, float b,
+                                                 float& d1, float& d2)
+{
+    TableRadial rad;
+    rad.t = tab;
+    complex_force_f32(rad, zn, a, b, d1, d2);
+}
+
 // Complex field phi1 + i phi2: V'(rho) from the radial table projected on phi / rho plus the
-// Z_N term (complex_force, latticeSimComplex_numba.py:2130-2152 and the same lines of the two
-// integrators); noise (cos, sin) of one hash draw per site.
+// Z_N term (latticeSimComplex_numba.py:2130-2152 and the same lines of the two integrators);
+// noise (cos, sin) of one hash draw per site.
 template <typename R, typename Tab, int OP, int BY>
 __global__ __launch_bounds__(64 * BY) void k_langevin_complex(
     const R* __restrict__ phi1, const R* __restrict__ phi2, const R* pi1, const R* pi2,
@@ -180,7 +199,7 @@ __global__ __launch_bounds__(64 * BY) void k_langevin_complex(
         const R lap1 = (s1 - (R)6 * ac) * c.inv_dx2;
         const R lap2 = (s2 - (R)6 * bc) * c.inv_dx2;
         R dV1, dV2;
-        complex_force(tab, zn, ac, bc, dV1, dV2);
+        lv_complex_force(tab, zn, ac, bc, dV1, dV2);
         const R F1 = c.inv_a2 * lap1 - dV1 * c.inv_mu2;
         const R F2 = c.inv_a2 * lap2 - dV2 * c.inv_mu2;
         const R F1p = c.inv_a2_prev * lap1 - dV1 * c.inv_mu2;

@@ -12,8 +12,9 @@
 // the root flags numbers the roots 1..n in raster order, which is scipy's numbering.
 //
 // Statistics: one thread per (x, y) row walks z and accumulates runs of equal label in
-// registers, flushing each run with one set of atomics (a percolating cluster costs one flush
-// per row, not one per site).  Coordinates are summed as exact 64-bit integers (sum x, sum
+// registers; each run is added to a per-block table in LDS, and the table is added to the
+// global sums once per block (a percolating cluster costs one set of global atomics per
+// block, not one per run).  Coordinates are summed as exact 64-bit integers (sum x, sum
 // x^2 over the three axes); the value sum is float64.
 #include <hip/hip_runtime.h>
 
@@ -209,55 +210,107 @@ struct CclStatsOut {
                                  // their bit patterns)
 };
 
-template <typename R>
-__device__ void flush(const CclStatsOut& o, int64_t nlab, int32_t lab, unsigned long long cnt,
-                      unsigned long long sx, unsigned long long sy, unsigned long long sz,
-                      unsigned long long sq, double sv, double pk)
+__device__ __forceinline__ void flush_global(const CclStatsOut& o, int64_t c,
+                                             unsigned long long cnt, unsigned long long sx,
+                                             unsigned long long sy, unsigned long long sz,
+                                             unsigned long long sq, double sv,
+                                             unsigned long long pk)
 {
-    if (lab <= 0 || lab > nlab) return;
-    const int64_t c = lab - 1;
     atomicAdd(o.volume + c, cnt);
     atomicAdd(o.sx + 3 * c, sx);
     atomicAdd(o.sx + 3 * c + 1, sy);
     atomicAdd(o.sx + 3 * c + 2, sz);
     atomicAdd(o.sxx + c, sq);
     atomicAdd(o.sum_v + c, sv);
-    atomicMax(o.peak + c, (unsigned long long)__double_as_longlong(pk));
+    atomicMax(o.peak + c, pk);
 }
+
+// Per-block accumulators for the labels met by the block's rows: an open-addressing table in
+// LDS (key 0 = empty), so a large component costs one set of global atomics per block instead
+// of one per run.  A run whose label finds no slot within kProbe probes goes to global memory
+// directly.
+constexpr int kSlots = 512;
+constexpr int kProbe = 32;
+
+struct StatsTable {
+    int32_t key[kSlots];
+    unsigned long long cnt[kSlots], sx[kSlots], sy[kSlots], sz[kSlots], sq[kSlots], pk[kSlots];
+    double sv[kSlots];
+};
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_ccl_stats(const R* __restrict__ v,
                                                    const int32_t* __restrict__ L, int nx, int ny,
                                                    int nz, int64_t nlab, CclStatsOut o)
 {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= (int64_t)nx * ny) return;
-    const unsigned long long x = (unsigned long long)(row / ny);
-    const unsigned long long y = (unsigned long long)(row % ny);
-    const int64_t base = row * nz;
-    int32_t cur = 0;
-    unsigned long long cnt = 0, sz = 0, sq = 0;
-    double sv = 0.0, pk = 0.0;
-    for (int k = 0; k < nz; ++k) {
-        const int32_t lab = L[base + k];
-        if (lab != cur) {
-            if (cnt) flush<R>(o, nlab, cur, cnt, cnt * x, cnt * y, sz, sq, sv, pk);
-            cur = lab;
-            cnt = sz = sq = 0;
-            sv = 0.0;
-            pk = 0.0;
-        }
-        if (lab > 0) {
-            const unsigned long long z = (unsigned long long)k;
-            const double val = (double)v[base + k];
-            ++cnt;
-            sz += z;
-            sq += x * x + y * y + z * z;
-            sv += val;
-            pk = fmax(pk, fabs(val));
-        }
+    __shared__ StatsTable tb;
+    for (int e = threadIdx.x; e < kSlots; e += blockDim.x) {
+        tb.key[e] = 0;
+        tb.cnt[e] = tb.sx[e] = tb.sy[e] = tb.sz[e] = tb.sq[e] = tb.pk[e] = 0;
+        tb.sv[e] = 0.0;
     }
-    if (cnt) flush<R>(o, nlab, cur, cnt, cnt * x, cnt * y, sz, sq, sv, pk);
+    __syncthreads();
+    auto flush = [&](int32_t lab, unsigned long long cnt, unsigned long long sx,
+                     unsigned long long sy, unsigned long long sz, unsigned long long sq,
+                     double sv, unsigned long long pk) {
+        if (lab <= 0 || lab > nlab) return;
+        unsigned h = ((unsigned)lab * 2654435761u) & (kSlots - 1);
+        for (int p = 0; p < kProbe; ++p) {
+            const int32_t old = atomicCAS(&tb.key[h], 0, lab);
+            if (old == 0 || old == lab) {
+                atomicAdd(&tb.cnt[h], cnt);
+                atomicAdd(&tb.sx[h], sx);
+                atomicAdd(&tb.sy[h], sy);
+                atomicAdd(&tb.sz[h], sz);
+                atomicAdd(&tb.sq[h], sq);
+                atomicAdd(&tb.sv[h], sv);
+                atomicMax(&tb.pk[h], pk);
+                return;
+            }
+            h = (h + 1) & (kSlots - 1);
+        }
+        flush_global(o, lab - 1, cnt, sx, sy, sz, sq, sv, pk);
+    };
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < (int64_t)nx * ny) {
+        const unsigned long long x = (unsigned long long)(row / ny);
+        const unsigned long long y = (unsigned long long)(row % ny);
+        const int64_t base = row * nz;
+        int32_t cur = 0;
+        unsigned long long cnt = 0, sz = 0, sq = 0;
+        double sv = 0.0, pk = 0.0;
+        for (int k = 0; k < nz; ++k) {
+            const int32_t lab = L[base + k];
+            if (lab != cur) {
+                if (cnt)
+                    flush(cur, cnt, cnt * x, cnt * y, sz, sq, sv,
+                          (unsigned long long)__double_as_longlong(pk));
+                cur = lab;
+                cnt = sz = sq = 0;
+                sv = 0.0;
+                pk = 0.0;
+            }
+            if (lab > 0) {
+                const unsigned long long z = (unsigned long long)k;
+                const double val = (double)v[base + k];
+                ++cnt;
+                sz += z;
+                sq += x * x + y * y + z * z;
+                sv += val;
+                pk = fmax(pk, fabs(val));
+            }
+        }
+        if (cnt)
+            flush(cur, cnt, cnt * x, cnt * y, sz, sq, sv,
+                  (unsigned long long)__double_as_longlong(pk));
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSlots; e += blockDim.x) {
+        const int32_t lab = tb.key[e];
+        if (lab > 0)
+            flush_global(o, lab - 1, tb.cnt[e], tb.sx[e], tb.sy[e], tb.sz[e], tb.sq[e], tb.sv[e],
+                         tb.pk[e]);
+    }
 }
 
 unsigned grid_for(int64_t n)
