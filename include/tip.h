@@ -357,13 +357,20 @@ int tip_lattice_step(tip_lattice *lat, const tip_step_params *p, int64_t nsteps,
 /* The overdamped and BAOAB integrators on the lattice (tip_overdamped_*, tip_baoab_*), with
  * eta_eff as eta (overdamped) or gamma (BAOAB) and noise_scale as the integrator's own
  * amplitude: overdamped sqrt(2 T dt / (eta_eff mu^2 dx^3)), BAOAB sqrt(T / (mu^2 dx^3))
- * (latticeSimComplex_numba.py:4293, 4313).  A BAOAB call of nsteps runs OPEN, FUSED x
- * (nsteps - 1), CLOSE, so pi is the true momentum between calls; the overdamped step leaves
- * pi untouched. */
+ * (latticeSimComplex_numba.py:4293, 4313).  A BAOAB call of nsteps runs OPEN (or FUSED,
+ * completing the previous call's last step) and FUSED x (nsteps - 1): one lattice pass per
+ * step.  The closing half-kick of its last step is deferred: the next BAOAB call fuses it
+ * (when dt, dx and mu agree; its inv_a2 stays that of the step it belongs to), and any
+ * operation that needs the true pi or changes the force runs it first with the old
+ * parameters -- tip_lattice_download / _field_ptr of a momentum, _upload, _set_table,
+ * _set_zn, the other steppers and tip_lattice_finish.  Field reads of phi (vacuum count,
+ * winding, min/max, rho/theta) do not.  The overdamped step leaves pi untouched. */
 int tip_lattice_step_overdamped(tip_lattice *lat, const tip_step_params *p, int64_t nsteps,
                                 uint64_t step0, uint64_t seed, float *elapsed_ms);
 int tip_lattice_step_baoab(tip_lattice *lat, const tip_step_params *p, int64_t nsteps,
                            uint64_t step0, uint64_t seed, float *elapsed_ms);
+/* Complete deferred work (the BAOAB closing half-kick) and synchronise the lattice stream. */
+int tip_lattice_finish(tip_lattice *lat);
 /* Synchronous observables on the current state. */
 int tip_lattice_vacuum_count(tip_lattice *lat, double esc, unsigned long long *count);
 /* plaq_dev (int8 [3][N]) and wsum_dev (float [N], the per-site winding density) are nullable

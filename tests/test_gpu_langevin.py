@@ -8,9 +8,10 @@ Bars:
   * float64 drop-ins (real; complex with Z_N on): fields within 1e-12 relative of the oracle
     after 6 steps with hash noise (the only differences are libm's log/cos/sin/atan2)
   * float32 drop-ins: within 2e-5 relative of the float64 oracle after 6 steps
-  * lattice BAOAB over k steps (open, fused x (k-1), close): float64 bit-identical to k
-    drop-in steps; float32 within 1e-6 relative; the overdamped lattice path equals the
-    drop-in loop and leaves pi untouched
+  * lattice BAOAB over k steps (open, fused x (k-1), deferred close): float64 bit-identical
+    to k drop-in steps, also as k single-step calls across table and inv_a2 changes; float32
+    within 1e-6 relative; the overdamped lattice path equals the drop-in loop and leaves pi
+    untouched
   * the driver's complex overdamped and BAOAB loops replay on the oracle fed the same tables
     (float32 bar 2e-5)
 """
@@ -160,6 +161,43 @@ def test_lattice_baoab_fused_equals_steps(dtype):
     else:
         assert rel(lp.astype(np.float64), phi.astype(np.float64)) < 1e-6
         assert rel(lq.astype(np.float64), pi.astype(np.float64)) < 1e-6
+
+
+def test_lattice_baoab_deferred_kick():
+    """Single-step BAOAB calls fuse the deferred half-kick into the next call: k calls of one
+    step equal one call of k steps and the drop-in loop (float64, bitwise), also across a
+    table change (the pending kick runs with the old table first) and an inv_a2 change
+    (the fused pass takes both); reading phi does not complete the kick, reading pi does."""
+    from tip import kernels as K
+    from tip.lattice import Lattice
+    shape = (12, 10, 70)
+    phi0, pi0, tab, prm = real_setup(shape, 11)
+    tab2 = (tab[0] * 1.01, tab[1], tab[2])
+    kw = dict(dt=prm["dt"], dx=prm["dx"], gamma=prm["eta"], mu=prm["mu"],
+              noise_scale=prm["ns"], seed=0)
+    inv = [0.93, 0.93, 0.95, 0.97, 0.97, 0.97]
+    lat = Lattice(shape, np.float64)
+    try:
+        lat.upload("phi", phi0)
+        lat.upload("pi", pi0)
+        lat.set_table(*tab, odd=True)
+        for n in range(6):
+            if n == 4:
+                lat.set_table(*tab2, odd=True)
+            lat.step_baoab(1, inv_a2=inv[n], step0=n, **kw)
+            if n == 2:
+                mid_phi = lat.download("phi")  # no completion
+        lp, lq = lat.download("phi"), lat.download("pi")
+    finally:
+        lat.close()
+    phi, pi = phi0.copy(), pi0.copy()
+    for n in range(6):
+        t = tab2 if n >= 4 else tab
+        K.baoab_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta"], prm["mu"], inv[n],
+                           t[0], t[1], t[2], len(t[0]), prm["ns"], O.noise_seed(n), odd=True)
+        if n == 2:
+            assert np.array_equal(mid_phi, phi)
+    assert np.array_equal(lp, phi) and np.array_equal(lq, pi)
 
 
 def test_lattice_complex_integrators_match_drop_in():

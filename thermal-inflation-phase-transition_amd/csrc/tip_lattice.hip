@@ -23,11 +23,17 @@ struct tip_lattice {
     tip::StepGeometry geom = {0, 0, 1, TIP_KERNEL_AUTO};  // defaults for the lattice size
     float2* mlp_grid = nullptr;  // per-step grid of the surrogate ((npts - 1) float2)
     int64_t mlp_grid_cap = 0;
+    // BAOAB: the closing half-kick of the last step is deferred (pi holds the post-O
+    // momentum) until the next BAOAB call fuses it or an operation needs the true pi.
+    int baoab_pending = 0;
+    tip_step_params baoab_prev = {};
 };
 
 using namespace tip;
 
 namespace {
+
+int close_baoab(tip_lattice* L);
 
 size_t field_bytes(const tip_lattice* L) { return (size_t)(L->nx * L->ny * L->nz) * L->prec; }
 
@@ -115,6 +121,8 @@ extern "C" int tip_lattice_field_ptr(tip_lattice* lat, int which, void** ptr)
     TIP_REQUIRE(lat && ptr, "tip_lattice_field_ptr: NULL");
     int s = slot_of(lat, which);
     TIP_REQUIRE(s >= 0, "tip_lattice_field_ptr: bad field index %d", which);
+    int rc = close_baoab(lat);  // the caller may read or write any field through the pointer
+    if (rc) return rc;
     *ptr = lat->buf[lat->cur][s];
     return TIP_OK;
 }
@@ -124,6 +132,8 @@ extern "C" int tip_lattice_upload(tip_lattice* lat, int which, const void* host)
     TIP_REQUIRE(lat && host, "tip_lattice_upload: NULL");
     int s = slot_of(lat, which);
     TIP_REQUIRE(s >= 0, "tip_lattice_upload: bad field index %d", which);
+    int rc = close_baoab(lat);
+    if (rc) return rc;
     TIP_HIP(hipMemcpyAsync(lat->buf[lat->cur][s], host, field_bytes(lat), hipMemcpyHostToDevice,
                            lat->stream));
     TIP_HIP(hipStreamSynchronize(lat->stream));
@@ -135,6 +145,11 @@ extern "C" int tip_lattice_download(tip_lattice* lat, int which, void* host)
     TIP_REQUIRE(lat && host, "tip_lattice_download: NULL");
     int s = slot_of(lat, which);
     TIP_REQUIRE(s >= 0, "tip_lattice_download: bad field index %d", which);
+    const bool momentum = lat->kind == TIP_REAL ? which == 1 : (which == 1 || which == 3);
+    if (momentum) {
+        int rc = close_baoab(lat);
+        if (rc) return rc;
+    }
     TIP_HIP(hipMemcpyAsync(host, lat->buf[lat->cur][s], field_bytes(lat), hipMemcpyDeviceToHost,
                            lat->stream));
     TIP_HIP(hipStreamSynchronize(lat->stream));
@@ -148,6 +163,8 @@ extern "C" int tip_lattice_set_table(tip_lattice* lat, const double* host_table,
     TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "tip_lattice_set_table: npts out of range");
     TIP_REQUIRE(dinv > 0.0, "tip_lattice_set_table: dinv must be positive");
     TIP_REQUIRE(!(lat->kind == TIP_COMPLEX && odd), "complex tables are tabulated on rho >= 0");
+    int rc0 = close_baoab(lat);  // the deferred half-kick belongs to the old table
+    if (rc0) return rc0;
     if (npts > lat->cap) {
         TIP_HIP(hipStreamSynchronize(lat->stream));
         if (lat->table) TIP_HIP(hipFree(lat->table));
@@ -173,6 +190,8 @@ extern "C" int tip_lattice_set_table(tip_lattice* lat, const double* host_table,
 extern "C" int tip_lattice_set_zn(tip_lattice* lat, double zn_n, double zn_dv, double zn_active)
 {
     TIP_REQUIRE(lat, "tip_lattice_set_zn: NULL");
+    int rc = close_baoab(lat);
+    if (rc) return rc;
     lat->zn_n = zn_n;
     lat->zn_dv = zn_dv;
     lat->zn_active = zn_active;
@@ -220,6 +239,8 @@ extern "C" int tip_lattice_step(tip_lattice* L, const tip_step_params* p, int64_
     TIP_REQUIRE(L && p, "tip_lattice_step: NULL");
     TIP_REQUIRE(L->npts >= 2, "tip_lattice_step: no V' table set");
     TIP_REQUIRE(nsteps >= 0, "tip_lattice_step: nsteps < 0");
+    int rc0 = close_baoab(L);
+    if (rc0) return rc0;
     const StepScalars<float> c32 =
         make_scalars<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, p->noise_scale);
     const SegTable tab32 = make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd);
@@ -320,13 +341,33 @@ int langevin_checks(tip_lattice* L, const tip_step_params* p, int64_t nsteps, co
     return TIP_OK;
 }
 
+// Run the deferred closing half-kick of the last BAOAB step (with that step's parameters and
+// the table and Z_N parameters still in place).
+int close_baoab(tip_lattice* L)
+{
+    if (!L->baoab_pending) return TIP_OK;
+    L->baoab_pending = 0;
+    return lattice_langevin(L, LV_BAOAB_CLOSE, &L->baoab_prev, L->baoab_prev.inv_a2, 0);
+}
+
 }  // namespace
+
+extern "C" int tip_lattice_finish(tip_lattice* L)
+{
+    TIP_REQUIRE(L, "tip_lattice_finish: NULL");
+    int rc = close_baoab(L);
+    if (rc) return rc;
+    TIP_HIP(hipStreamSynchronize(L->stream));
+    return TIP_OK;
+}
 
 extern "C" int tip_lattice_step_overdamped(tip_lattice* L, const tip_step_params* p,
                                            int64_t nsteps, uint64_t step0, uint64_t seed,
                                            float* elapsed_ms)
 {
     int rc = langevin_checks(L, p, nsteps, "tip_lattice_step_overdamped");
+    if (rc) return rc;
+    rc = close_baoab(L);
     if (rc) return rc;
     if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
     for (int64_t n = 0; n < nsteps; ++n) {
@@ -349,15 +390,27 @@ extern "C" int tip_lattice_step_baoab(tip_lattice* L, const tip_step_params* p, 
     int rc = langevin_checks(L, p, nsteps, "tip_lattice_step_baoab");
     if (rc) return rc;
     if (nsteps == 0) return TIP_OK;
+    // A deferred half-kick fuses into this call's first pass when the kick and drift
+    // constants agree (dt, dx, mu; inv_a2 may differ: the fused pass takes both).
+    const tip_step_params& q = L->baoab_prev;
+    const bool fuse_first =
+        L->baoab_pending && q.dt == p->dt && q.dx == p->dx && q.mu == p->mu;
+    if (L->baoab_pending && !fuse_first) {
+        rc = close_baoab(L);
+        if (rc) return rc;
+    }
+    const double inv_a2_prev = fuse_first ? q.inv_a2 : p->inv_a2;
     if (elapsed_ms) TIP_HIP(hipEventRecord(L->ev0, L->stream));
     for (int64_t n = 0; n < nsteps; ++n) {
-        rc = lattice_langevin(L, n == 0 ? LV_BAOAB_OPEN : LV_BAOAB_FUSED, p, p->inv_a2,
+        const bool fused = n > 0 || fuse_first;
+        rc = lattice_langevin(L, fused ? LV_BAOAB_FUSED : LV_BAOAB_OPEN, p,
+                              n == 0 ? inv_a2_prev : p->inv_a2,
                               tip_noise_seed(step0 + (uint64_t)n, seed));
         if (rc) return rc;
         L->cur ^= 1;
     }
-    rc = lattice_langevin(L, LV_BAOAB_CLOSE, p, p->inv_a2, 0);
-    if (rc) return rc;
+    L->baoab_pending = 1;
+    L->baoab_prev = *p;
     if (elapsed_ms) {
         TIP_HIP(hipEventRecord(L->ev1, L->stream));
         TIP_HIP(hipEventSynchronize(L->ev1));
@@ -372,6 +425,8 @@ extern "C" int tip_lattice_step_mlp(tip_lattice* L, const tip_step_params* p,
                                     uint64_t seed, int64_t grid_points, float* elapsed_ms)
 {
     TIP_REQUIRE(L && p && mlp_params && fm, "tip_lattice_step_mlp: NULL");
+    int rc0 = close_baoab(L);
+    if (rc0) return rc0;
     TIP_REQUIRE(L->kind == TIP_COMPLEX, "tip_lattice_step_mlp: needs a complex lattice");
     TIP_REQUIRE(nsteps >= 0, "tip_lattice_step_mlp: nsteps < 0");
     TIP_REQUIRE(grid_points == 0 || (grid_points >= 2 && grid_points <= (int64_t(1) << 24)),

@@ -105,6 +105,7 @@ SIGNATURES = {
                                            C.POINTER(C.c_float)]),
     "tip_lattice_step_baoab": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,
                                       C.POINTER(C.c_float)]),
+    "tip_lattice_finish": (_int, [_vp]),
     "tip_overdamped_f32": (_int, [_vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64, _int, _i64,
                                   _i64, _i64, _u64, _d, _vp]),
     "tip_overdamped_f64": (_int, [_vp, _vp, _d, _d, _d, _d, _d, _vp, _d, _d, _i64, _int, _i64,

@@ -110,10 +110,16 @@ class Lattice:
     def step_baoab(self, nsteps: int, *, dt: float, dx: float, gamma: float, mu: float,
                    inv_a2: float = 1.0, noise_scale: float = 0.0, seed: int = 0,
                    step0: Optional[int] = None, timed: bool = False) -> Optional[float]:
-        """Advance nsteps of BAOAB (baoab_step_table) in nsteps + 1 lattice passes;
-        noise_scale = sqrt(T / (mu^2 dx^3))."""
+        """Advance nsteps of BAOAB (baoab_step_table), one lattice pass per step;
+        noise_scale = sqrt(T / (mu^2 dx^3)).  The last step's closing half-kick is deferred to
+        the next BAOAB call or to the first operation that needs pi (download of a momentum,
+        upload, set_table, set_zn, another stepper, finish())."""
         return self._langevin("tip_lattice_step_baoab", nsteps, dt, dx, gamma, mu, inv_a2,
                               noise_scale, seed, step0, timed)
+
+    def finish(self) -> None:
+        """Complete deferred work (the BAOAB closing half-kick) and synchronise."""
+        _lib.call("tip_lattice_finish", self._h)
 
     def step_mlp(self, nsteps: int, surrogate, *, T_start: float, dT_per_step: float = 0.0,
                  dt: float, dx: float, eta_eff: float, mu: float, inv_a2: float = 1.0,
