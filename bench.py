@@ -100,6 +100,36 @@ def cpu_baseline(shape, table, tmin, dinv, prm, budget_s: float = 12.0):
                 ms_per_step=1e3 * el / steps)
 
 
+def cpu_baseline_integrators(table, tmin, dinv, prm, n: int = 128, reps: int = 3) -> dict:
+    """The oracle's BAOAB (open + close) and overdamped steps (float64, OpenMP) at n^3 on the
+    host cores, for the BAOAB / overdamped lines of the extras."""
+    from oracle import oracle as O
+
+    shape = (n, n, n)
+    rng = np.random.default_rng(3)
+    phi = 0.01 * rng.normal(size=shape)
+    pi = np.zeros(shape)
+    out = {"cores": O.num_threads(), "kind": "port", "unit": "site-updates/s"}
+    for kind in ("baoab", "overdamped"):
+        def one(k):
+            if kind == "baoab":
+                return O.baoab_step(phi, pi, prm["dt"], prm["dx"], prm["eta_eff"], prm["mu"],
+                                    1.0, table, tmin, dinv, odd=True, noise_seed=k,
+                                    noise_scale=30.0)
+            return O.langevin(O.OVERDAMPED, phi, None, prm["dt"], prm["dx"], prm["eta_eff"],
+                              prm["mu"], 1.0, table, tmin, dinv, odd=True, noise_seed=k,
+                              noise_scale=30.0)
+        one(0)
+        t0 = time.perf_counter()
+        for k in range(reps):
+            one(k + 1)
+        el = (time.perf_counter() - t0) / reps
+        out[kind] = {"value": n ** 3 / el, "ms_per_step": 1e3 * el,
+                     "sample": f"{reps} steps of {n}^3 real float64 (oracle restatement)"}
+        log(f"cpu {kind} {n}^3: {1e3 * el:.1f} ms/step")
+    return out
+
+
 def ensemble_workload(npts: int = 8192, seed: int = 2024):
     """64 realisations: 32 x 64^3, 24 x 96^3, 8 x 128^3; seeds 1..64; T0 ~ U[6770, 7350] GeV;
     per-instance 8192-point float64 V' tables on [0, 1.5 phi_true(T0)] (odd)."""
@@ -441,6 +471,9 @@ def main() -> int:
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)
+        if "extras" in out:
+            out["extras"]["cpu_baseline_integrators"] = cpu_baseline_integrators(
+                table, tmin, dinv, prm)
     print(json.dumps(out), flush=True)
     return 0
 

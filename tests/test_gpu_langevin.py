@@ -285,3 +285,38 @@ def test_driver_complex_integrators_replay_on_oracle(integrator):
                                             t, tmin, dinv, zn=zn)
     for g, r in zip(got, (a, b, u, v)):
         assert rel(g, r) < 2e-5
+
+
+def test_full_size_noise_statistics():
+    """256^3 float32 with the force switched off (zero table, dx so large the Laplacian
+    vanishes): the BAOAB momentum is an Ornstein-Uhlenbeck process, var(pi) after n steps =
+    ns^2 (1 - c1^(2n)), c1 = exp(-gamma dt); the overdamped field is a random walk, var(phi)
+    = n ns^2, with independent draws per step (lag-one correlation of the increments ~ 0).
+    Bars: 1 % on the variances (the sampling error at 2^24 sites is 0.03 %)."""
+    from tip.lattice import Lattice
+    shape = (256, 256, 256)
+    zero = np.zeros(shape, np.float32)
+    tab = (np.zeros(9), 0.0, 1.0)
+    ns, gamma, dt = 0.7, 0.4, 0.25
+    lat = Lattice(shape, np.float32)
+    try:
+        lat.upload("phi", zero)
+        lat.upload("pi", zero)
+        lat.set_table(*tab)
+        n = 12
+        lat.step_baoab(n, dt=dt, dx=1e15, gamma=gamma, mu=1.0, noise_scale=ns, seed=3)
+        pi = lat.download("pi").astype(np.float64)
+        want = ns ** 2 * (1.0 - math.exp(-2.0 * gamma * dt * n))
+        assert abs(pi.var() / want - 1.0) < 0.01 and abs(pi.mean()) < 0.01 * ns
+        lat.upload("phi", zero)
+        lat.step_overdamped(1, dt=dt, dx=1e15, eta_eff=1.0, mu=1.0, noise_scale=ns, seed=5,
+                            step0=0)
+        d1 = lat.download("phi").astype(np.float64)
+        lat.step_overdamped(7, dt=dt, dx=1e15, eta_eff=1.0, mu=1.0, noise_scale=ns, seed=5,
+                            step0=1)
+        d8 = lat.download("phi").astype(np.float64)
+        assert abs(d1.var() / ns ** 2 - 1.0) < 0.01 and abs(d8.var() / (8 * ns ** 2) - 1.0) < 0.01
+        inc = d8 - d1
+        assert abs(np.corrcoef(d1.ravel()[:1 << 22], inc.ravel()[:1 << 22])[0, 1]) < 3e-3
+    finally:
+        lat.close()
