@@ -320,3 +320,43 @@ def test_full_size_noise_statistics():
         assert abs(np.corrcoef(d1.ravel()[:1 << 22], inc.ravel()[:1 << 22])[0, 1]) < 3e-3
     finally:
         lat.close()
+
+
+@pytest.mark.parametrize("complex_field", [False, True])
+def test_lattice_integrator_geometries(complex_field):
+    """The row tiles (1, 2, 4 y rows per thread, several plane counts) and the flat tile give
+    the same BAOAB and overdamped fields to float32 rounding (the two forms differ only in the
+    table index arithmetic at segment boundaries)."""
+    from tip.lattice import Lattice
+    shape = (20, 16, 96)
+    if complex_field:
+        f, q, tab, prm = complex_setup(shape, 13)
+        init = dict(phi1=f[0], phi2=f[1], pi1=q[0], pi2=q[1])
+    else:
+        phi0, pi0, tab, prm = real_setup(shape, 13)
+        init = dict(phi=phi0, pi=pi0)
+    init = {k: v.astype(np.float32) for k, v in init.items()}
+    kw = dict(dt=prm["dt"], dx=prm["dx"], mu=prm["mu"], inv_a2=prm["inv_a2"],
+              noise_scale=prm["ns"], seed=2, step0=0)
+    out = {}
+    for geo in [(0, 0, 1, 2), (1, 0, 1, 1), (2, 3, 0, 1), (4, 7, 1, 1), (2, 40, 1, 1)]:
+        for op in ("baoab", "overdamped"):
+            lat = Lattice(shape, np.float32, complex_field=complex_field)
+            try:
+                for k, v in init.items():
+                    lat.upload(k, v)
+                lat.set_table(*tab, odd=not complex_field)
+                if complex_field:
+                    lat.set_zn(3.0, 1e8, 1.0)
+                lat.set_geometry(*geo)
+                if op == "baoab":
+                    lat.step_baoab(5, gamma=prm["eta"], **kw)
+                else:
+                    lat.step_overdamped(5, eta_eff=prm["eta"], **kw)
+                out[geo, op] = {k: lat.download(k).astype(np.float64) for k in init}
+            finally:
+                lat.close()
+    for (geo, op), o in out.items():
+        ref = out[(0, 0, 1, 2), op]
+        for k in init:
+            assert rel(o[k], ref[k]) < 1e-6, (geo, op, k)

@@ -231,6 +231,219 @@ __global__ __launch_bounds__(64 * BY) void k_langevin_complex(
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// float32 row tiles for the two integrators (the default where the shape allows): the tile
+// and LDS scheme of the RK2 row kernel, with only phi staged -- the force needs the stencil of
+// phi and pi only at the site.  NF = 1 (real) or 2 (complex) stencil fields; a group of
+// W = ceil(nz/64) waves covers whole z rows, each thread TY consecutive y rows; per plane the
+// group writes its rows of phi to LDS (K = 2 NF TY + 1 words per z column: two buffers, odd
+// stride) behind one barrier; z neighbours are LDS reads, y neighbours registers except the
+// two halo rows, x neighbours the register window (x-1, x, x+1, and x+2 in flight).  Loads
+// per site: phi at x+2, pi at x+1 (BAOAB) and 2/TY halo rows per field.
+// ---------------------------------------------------------------------------------------
+template <int NF, int TY, int OP, bool ODD>
+__device__ __forceinline__ void lv_rows_tile(
+    const float* const (&phi)[NF], const float* const (&pi)[NF], float* const (&phi_out)[NF],
+    float* const (&pi_out)[NF], const LangevinScalars<float>& c, const TableF32<ODD>& tab,
+    const ZnParams<float>& zn, int nx, int ny, int nz, int W, int tile, int x0, int x1,
+    uint64_t noise_seed, int hash, float* __restrict__ lds)
+{
+    constexpr int K = 2 * NF * TY + 1;
+    constexpr bool MOM = OP != LV_OVERDAMPED;  // reads and writes pi
+    const int G = (int)blockDim.y / W;
+    const int w = (int)threadIdx.y, grp = w / W, zw = w - grp * W;
+    const int z = zw * 64 + (int)threadIdx.x;
+    const bool zact = z < nz;
+    const int zc = zact ? z : 0;
+    const int zp = zc + 1 < nz ? zc + 1 : 0, zm = zc > 0 ? zc - 1 : nz - 1;
+    const int y0 = (tile * G + grp) * TY;
+    const bool act = zact && y0 < ny;
+    const int yb = y0 < ny ? y0 : 0;
+    float* Lg = lds + (size_t)grp * (64 * W) * K;
+    float* Lz = Lg + z * K;
+    const float* Lzp = Lg + zp * K;
+    const float* Lzm = Lg + zm * K;
+
+    const int nyz = ny * nz;
+    const size_t plane = (size_t)nyz * 4u;
+    const uint32_t pb = (uint32_t)plane;
+    auto at = [&](const float* b, int x) {
+        return make_rsrc(reinterpret_cast<const char*>(b) + (size_t)x * plane, pb);
+    };
+    const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
+    uint32_t off[TY], offs[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        off[r] = (uint32_t)((yb + r) * nz + zc) * 4u;
+        offs[r] = act ? off[r] : 0xFFFFFFF0u;  // idle lanes store past the end (dropped)
+    }
+    const uint32_t offm = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz + zc) * 4u;
+    const uint32_t offp = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz + zc) * 4u;
+    const int64_t col0 = (int64_t)yb * nz + zc;
+
+    // phi window (x-1, x, x+1, x+2 in the four role slots), pi at x (and x+1 in flight), halo
+    // rows y0-1 and y0+TY of phi for the current / next plane.
+    float w0[NF][TY], w1[NF][TY], w2[NF][TY], w3[NF][TY];
+    float m0[NF][TY], m1[NF][TY];
+    float hA[NF][2], hB[NF][2];
+    {
+        const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
+        const int xq = x0 + 1 < nx ? x0 + 1 : 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                w0[f][r] = buf_load<float>(at(phi[f], xm), off[r]);
+                w1[f][r] = buf_load<float>(at(phi[f], x0), off[r]);
+                w2[f][r] = buf_load<float>(at(phi[f], xq), off[r]);
+                w3[f][r] = 0.0f;
+                m0[f][r] = MOM ? buf_load<float>(at(pi[f], x0), off[r]) : 0.0f;
+                m1[f][r] = 0.0f;
+            }
+            hA[f][0] = buf_load<float>(at(phi[f], x0), offm);
+            hA[f][1] = buf_load<float>(at(phi[f], x0), offp);
+            hB[f][0] = hB[f][1] = 0.0f;
+        }
+    }
+
+    auto plane_step = [&](auto bufc, int x, const float (&pm)[NF][TY], const float (&pc)[NF][TY],
+                          const float (&pp)[NF][TY], float (&pn)[NF][TY],
+                          const float (&qc)[NF][TY], float (&qn)[NF][TY],
+                          const float (&hc)[NF][2], float (&hn)[NF][2]) {
+        constexpr int b = decltype(bufc)::value;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int r = 0; r < TY; ++r) Lz[(b * NF + f) * TY + r] = pc[f][r];
+        {
+            const int xa = x + 1 < nx ? x + 1 : 0;
+            const int xb = xa + 1 < nx ? xa + 1 : 0;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                for (int r = 0; r < TY; ++r) {
+                    pn[f][r] = buf_load<float>(at(phi[f], xb), off[r]);
+                    if (MOM) qn[f][r] = buf_load<float>(at(pi[f], xa), off[r]);
+                }
+                hn[f][0] = buf_load<float>(at(phi[f], xa), offm);
+                hn[f][1] = buf_load<float>(at(phi[f], xa), offp);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            float F[NF], Fp[NF];
+            float lap[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const int o = (b * NF + f) * TY + r;
+                const float yp = r + 1 < TY ? pc[f][r + 1 < TY ? r + 1 : r] : hc[f][1];
+                const float ym = r > 0 ? pc[f][r > 0 ? r - 1 : 0] : hc[f][0];
+                float s1 = pp[f][r] + pm[f][r];
+                s1 = s1 + yp;
+                s1 = s1 + ym;
+                s1 = s1 + Lzp[o];
+                s1 = s1 + Lzm[o];
+                lap[f] = (s1 - 6.0f * pc[f][r]) * c.inv_dx2;
+            }
+            float dV[NF];
+            if constexpr (NF == 1) {
+                dV[0] = seg_eval<ODD>(seg_load<ODD>(T, tab, pc[0][r]), pc[0][r]);
+            } else {
+                TableRadial rad;
+                rad.t = tab;  // complex tables are even (ODD = false)
+                complex_force_f32(rad, zn, pc[0][r], pc[NF - 1][r], dV[0], dV[NF - 1]);
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                F[f] = c.inv_a2 * lap[f] - dV[f] * c.inv_mu2;
+                Fp[f] = c.inv_a2_prev * lap[f] - dV[f] * c.inv_mu2;
+            }
+            float zv[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) zv[f] = 0.0f;
+            if (OP != LV_BAOAB_CLOSE && hash) {
+                const uint64_t seed = site_seed((int64_t)x * nyz + col0 + (int64_t)r * nz,
+                                                noise_seed);
+                if constexpr (NF == 1) zv[0] = gauss_of(0.0f, seed, c.word_mask);
+                else gauss_pair_of(0.0f, seed, c.word_mask, zv[0], zv[NF - 1]);
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                float a_new = 0.0f, p_new = 0.0f;
+                langevin_site<float, OP>(c, pc[f][r], MOM ? qc[f][r] : 0.0f, F[f], Fp[f], zv[f],
+                                         a_new, p_new);
+                if (OP != LV_BAOAB_CLOSE) buf_store(a_new, at(phi_out[f], x), offs[r]);
+                if (MOM) buf_store(p_new, at(pi_out[f], x), offs[r]);
+            }
+        }
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    int x = x0;
+    for (; x + 4 <= x1; x += 4) {
+        plane_step(B0{}, x + 0, w0, w1, w2, w3, m0, m1, hA, hB);
+        plane_step(B1{}, x + 1, w1, w2, w3, w0, m1, m0, hB, hA);
+        plane_step(B0{}, x + 2, w2, w3, w0, w1, m0, m1, hA, hB);
+        plane_step(B1{}, x + 3, w3, w0, w1, w2, m1, m0, hB, hA);
+    }
+    // remainder: canonical roles with an explicit rotation; the buffers keep alternating
+    auto rotate = [&]() {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                w0[f][r] = w1[f][r];
+                w1[f][r] = w2[f][r];
+                w2[f][r] = w3[f][r];
+                m0[f][r] = m1[f][r];
+            }
+            hA[f][0] = hB[f][0];
+            hA[f][1] = hB[f][1];
+        }
+    };
+    if (x < x1) {
+        plane_step(B0{}, x, w0, w1, w2, w3, m0, m1, hA, hB);
+        rotate();
+        ++x;
+    }
+    if (x < x1) {
+        plane_step(B1{}, x, w0, w1, w2, w3, m0, m1, hA, hB);
+        rotate();
+        ++x;
+    }
+    if (x < x1) plane_step(B0{}, x, w0, w1, w2, w3, m0, m1, hA, hB);
+}
+
+template <int NF, int TY, int OP, bool ODD>
+__global__ __launch_bounds__(NF == 1 ? 1024 : 512) void k_langevin_rows(
+    const float* __restrict__ f0, const float* __restrict__ f1, const float* q0,
+    const float* q1, float* __restrict__ o0, float* __restrict__ o1, float* r0, float* r1,
+    LangevinScalars<float> c, TableF32<ODD> tab, ZnParams<float> zn, int nx, int ny, int nz,
+    int W, TileGrid g, uint64_t noise_seed, int hash)
+{
+    extern __shared__ float lds_lv[];
+    int tp, ch;
+    g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
+    const int x0 = ch * g.cx;
+    const int x1 = min(x0 + g.cx, nx);
+    if constexpr (NF == 1) {
+        const float* const phi[1] = {f0};
+        const float* const pi[1] = {q0};
+        float* const po[1] = {o0};
+        float* const qo[1] = {r0};
+        lv_rows_tile<1, TY, OP, ODD>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
+                                      noise_seed, hash, lds_lv);
+    } else {
+        const float* const phi[2] = {f0, f1};
+        const float* const pi[2] = {q0, q1};
+        float* const po[2] = {o0, o1};
+        float* const qo[2] = {r0, r1};
+        lv_rows_tile<2, TY, OP, ODD>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
+                                      noise_seed, hash, lds_lv);
+    }
+}
+
 // Host launchers (defined once per precision by the two translation units).
 template <typename R, typename Tab>
 int launch_langevin_real(int op, const R* phi, const R* pi, R* phi_out, R* pi_out,
@@ -293,14 +506,18 @@ namespace tip {
 
 // Precision-specific launchers used by the C-ABI entry points and the lattice
 // (tip_langevin_f32.hip, tip_langevin_f64.hip).
+// geom (nullable): TIP_KERNEL_FLAT selects the flat tile, otherwise the row tile where the
+// shape allows (rows per thread and planes per block from geom when set).
 int langevin_real_f32(int op, const float* phi, const float* pi, float* phi_out, float* pi_out,
                       const LangevinScalars<float>& c, const SegTable& tab, int64_t nx,
-                      int64_t ny, int64_t nz, uint64_t noise_seed, int hash, hipStream_t st);
+                      int64_t ny, int64_t nz, uint64_t noise_seed, int hash, hipStream_t st,
+                      const StepGeometry* geom);
 int langevin_complex_f32(int op, const float* phi1, const float* phi2, const float* pi1,
                          const float* pi2, float* phi1_out, float* phi2_out, float* pi1_out,
                          float* pi2_out, const LangevinScalars<float>& c, const SegTable& tab,
                          const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
-                         uint64_t noise_seed, int hash, hipStream_t st);
+                         uint64_t noise_seed, int hash, hipStream_t st,
+                         const StepGeometry* geom);
 int langevin_real_f64(int op, const double* phi, const double* pi, double* phi_out,
                       double* pi_out, const LangevinScalars<double>& c, const double* table,
                       double tmin, double dinv, int64_t npts, int odd, int64_t nx, int64_t ny,

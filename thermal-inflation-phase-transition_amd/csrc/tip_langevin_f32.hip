@@ -4,12 +4,89 @@
 
 using namespace tip;
 
+namespace {
+
+// Row-tile launch (see lv_rows_tile); `launched` is false when the shape does not fit the row
+// tile (nz beyond the block, rows per thread not dividing ny) and the caller falls back to the
+// flat tile.
+template <int NF, bool ODD>
+int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, const float* q1,
+                   float* o0, float* o1, float* r0, float* r1, const LangevinScalars<float>& c,
+                   const TableF32<ODD>& tab, const ZnParams<float>& zn, int64_t nx, int64_t ny,
+                   int64_t nz, uint64_t noise_seed, int hash, const StepGeometry* geom,
+                   hipStream_t st, bool& launched)
+{
+    launched = false;
+    int ty = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
+    if (NF == 2 && !(geom && geom->by) && ty > 2) ty = 2;
+    const RowShape rs = row_shape(ny, nz, ty, NF == 1 ? 16 : 8);
+    if (!rs.ok) return TIP_OK;
+    TileGrid tg;
+    tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
+    int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
+    if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
+    TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
+    tg.cx = (int)cx;
+    tg.chunks = (int)((nx + cx - 1) / cx);
+    const int64_t nb64 = (int64_t)tg.tiles * tg.chunks;
+    TIP_REQUIRE(nb64 < (int64_t(1) << 31), "geometry: too many blocks");
+    const int nb = (int)nb64;
+    tg.xcd_remap = ((geom ? geom->xcd : 1) && nb % 8 == 0) ? 1 : 0;
+    const size_t lds = (size_t)rs.G * (64 * rs.W) * (2 * NF * rs.TY + 1) * sizeof(float);
+    const dim3 blk(64, rs.W * rs.G);
+#define TIP_LV_ROWS(TYV, OPV)                                                                   \
+    hipLaunchKernelGGL((k_langevin_rows<NF, TYV, OPV, ODD>), dim3(nb), blk, lds, st, f0, f1, q0, \
+                       q1, o0, o1, r0, r1, c, tab, zn, (int)nx, (int)ny, (int)nz, rs.W, tg,       \
+                       noise_seed, hash)
+#define TIP_LV_ROWS_OP(TYV)                                              \
+    switch (op) {                                                        \
+        case LV_OVERDAMPED: TIP_LV_ROWS(TYV, LV_OVERDAMPED); break;      \
+        case LV_BAOAB_OPEN: TIP_LV_ROWS(TYV, LV_BAOAB_OPEN); break;      \
+        case LV_BAOAB_FUSED: TIP_LV_ROWS(TYV, LV_BAOAB_FUSED); break;    \
+        case LV_BAOAB_CLOSE: TIP_LV_ROWS(TYV, LV_BAOAB_CLOSE); break;    \
+        default: TIP_REQUIRE(false, "langevin: unknown stage %d", op);   \
+    }
+    switch (rs.TY) {
+        case 1: TIP_LV_ROWS_OP(1); break;
+        case 4: TIP_LV_ROWS_OP(4); break;
+        default: TIP_LV_ROWS_OP(2); break;
+    }
+#undef TIP_LV_ROWS_OP
+#undef TIP_LV_ROWS
+    TIP_HIP(hipGetLastError());
+    launched = true;
+    return TIP_OK;
+}
+
+bool want_rows(const StepGeometry* geom)
+{
+    return !(geom && geom->kind == TIP_KERNEL_FLAT);
+}
+
+}  // namespace
+
 namespace tip {
 
 int langevin_real_f32(int op, const float* phi, const float* pi, float* phi_out, float* pi_out,
                       const LangevinScalars<float>& c, const SegTable& tab, int64_t nx,
-                      int64_t ny, int64_t nz, uint64_t noise_seed, int hash, hipStream_t st)
+                      int64_t ny, int64_t nz, uint64_t noise_seed, int hash, hipStream_t st,
+                      const StepGeometry* geom)
 {
+    if (want_rows(geom)) {
+        bool done = false;
+        const ZnParams<float> zn{};
+        int rc = tab.odd
+            ? launch_lv_rows<1, true>(op, phi, nullptr, pi, nullptr, phi_out, nullptr, pi_out,
+                                      nullptr, c, tab.as<true>(), zn, nx, ny, nz, noise_seed,
+                                      hash, geom, st, done)
+            : launch_lv_rows<1, false>(op, phi, nullptr, pi, nullptr, phi_out, nullptr, pi_out,
+                                       nullptr, c, tab.as<false>(), zn, nx, ny, nz, noise_seed,
+                                       hash, geom, st, done);
+        if (rc || done) return rc;
+        TIP_REQUIRE(!(geom && geom->kind == TIP_KERNEL_ROWS),
+                    "langevin f32: the row kernel needs nz <= 1024 and rows per thread dividing "
+                    "ny");
+    }
     if (tab.odd)
         return launch_langevin_real<float, TableF32<true>>(op, phi, pi, phi_out, pi_out, c,
                                                            tab.as<true>(), nx, ny, nz,
@@ -23,8 +100,19 @@ int langevin_complex_f32(int op, const float* phi1, const float* phi2, const flo
                          const float* pi2, float* phi1_out, float* phi2_out, float* pi1_out,
                          float* pi2_out, const LangevinScalars<float>& c, const SegTable& tab,
                          const ZnParams<float>& zn, int64_t nx, int64_t ny, int64_t nz,
-                         uint64_t noise_seed, int hash, hipStream_t st)
+                         uint64_t noise_seed, int hash, hipStream_t st,
+                         const StepGeometry* geom)
 {
+    if (want_rows(geom)) {
+        bool done = false;
+        int rc = launch_lv_rows<2, false>(op, phi1, phi2, pi1, pi2, phi1_out, phi2_out, pi1_out,
+                                          pi2_out, c, tab.as<false>(), zn, nx, ny, nz,
+                                          noise_seed, hash, geom, st, done);
+        if (rc || done) return rc;
+        TIP_REQUIRE(!(geom && geom->kind == TIP_KERNEL_ROWS),
+                    "langevin c32: the row kernel needs nz <= 512 and rows per thread dividing "
+                    "ny");
+    }
     return launch_langevin_complex<float, TableF32<false>>(op, phi1, phi2, pi1, pi2, phi1_out,
                                                            phi2_out, pi1_out, pi2_out, c,
                                                            tab.as<false>(), zn, nx, ny, nz,
@@ -85,7 +173,7 @@ extern "C" int tip_overdamped_f32(const float* phi, float* phi_out, double dt, d
     return langevin_real_f32(LV_OVERDAMPED, phi, nullptr, phi_out, nullptr,
                              make_langevin<float>(dt, dx, eta_eff, mu, inv_a2, inv_a2,
                                                   noise_scale),
-                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st);
+                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st, nullptr);
 }
 
 extern "C" int tip_baoab_f32(int stage, const float* phi, const float* pi, float* phi_out,
@@ -105,7 +193,7 @@ extern "C" int tip_baoab_f32(int stage, const float* phi, const float* pi, float
     return langevin_real_f32(stage, phi, pi, phi_out, pi_out,
                              make_langevin<float>(dt, dx, gamma, mu, inv_a2, inv_a2_prev,
                                                   noise_scale),
-                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st);
+                             tab, nx, ny, nz, noise_seed, noise_scale != 0.0, st, nullptr);
 }
 
 extern "C" int tip_overdamped_c32(const float* phi1, const float* phi2, float* phi1_out,
@@ -128,7 +216,7 @@ extern "C" int tip_overdamped_c32(const float* phi1, const float* phi2, float* p
                                 make_langevin<float>(dt, dx, eta_eff, mu, inv_a2, inv_a2,
                                                      noise_scale),
                                 tab, make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
-                                noise_scale != 0.0, st);
+                                noise_scale != 0.0, st, nullptr);
 }
 
 extern "C" int tip_baoab_c32(int stage, const float* phi1, const float* phi2, const float* pi1,
@@ -152,5 +240,5 @@ extern "C" int tip_baoab_c32(int stage, const float* phi1, const float* phi2, co
                                 make_langevin<float>(dt, dx, gamma, mu, inv_a2, inv_a2_prev,
                                                      noise_scale),
                                 tab, make_zn32(zn_n, zn_dv, zn_active), nx, ny, nz, noise_seed,
-                                noise_scale != 0.0, st);
+                                noise_scale != 0.0, st, nullptr);
 }

@@ -300,7 +300,7 @@ int lattice_langevin(tip_lattice* L, int op, const tip_step_params* p, double in
                 make_langevin<float>(p->dt, p->dx, p->eta_eff, p->mu, p->inv_a2, inv_a2_prev,
                                      p->noise_scale),
                 make_table32(L->table_ab, L->tmin, L->dinv, L->npts, L->odd), L->nx, L->ny,
-                L->nz, noise_seed, hash, L->stream);
+                L->nz, noise_seed, hash, L->stream, &L->geom);
         }
         return langevin_real_f64(
             op, (const double*)in[0], (const double*)in[1], (double*)out[0], (double*)pi_out,
@@ -318,7 +318,7 @@ int lattice_langevin(tip_lattice* L, int op, const tip_step_params* p, double in
                              p->noise_scale),
         make_table32(L->table_ab, L->tmin, L->dinv, L->npts, 0),
         make_zn32(L->zn_n, L->zn_dv, L->zn_active), L->nx, L->ny, L->nz, noise_seed, hash,
-        L->stream);
+        L->stream, &L->geom);
 }
 
 // Exchange the phi slots of the two parities (the overdamped step keeps pi where it is).
