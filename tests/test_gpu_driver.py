@@ -138,3 +138,54 @@ def test_driver_snapshot_keys(tmp_path):
         assert float(z["rho_max"]) == float(rho.max())
         w_ref, _ = O.winding(p1, p2)
         assert np.array_equal(z["winding"], np.rint(w_ref).astype(np.float32))
+
+
+@pytest.mark.parametrize("integrator", ["baoab", "overdamped"])
+def test_driver_real_f64_other_integrators_replay(integrator):
+    """The real float64 driver with Hubble expansion (inv_a2, eta_eff and T change every step)
+    under BAOAB and overdamped: fields within 1e-12 of the oracle fed the same tables and
+    schedule (BAOAB's deferred half-kick keeps each step's own inv_a2)."""
+    from tip import driver as D
+    from tip import potential as P
+    cfg = D.RunConfig(field="real", Nx=24, Ny=20, Nz=32, Nt=24, T0=7000.0, precision="f64",
+                      no_hubble=False, report=8, seed=4, quiet=True, table_size=4096,
+                      integrator=integrator)
+    run = D.Run(cfg)
+    tables = []
+    orig = run.lat.set_table
+
+    def rec(table, tmin, dinv, odd=False):
+        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
+        orig(table, tmin, dinv, odd=odd)
+
+    run.lat.set_table = rec
+    try:
+        run.run()
+        phi_g, pi_g = run.lat.download("phi"), run.lat.download("pi")
+    finally:
+        run.close()
+    f0 = D.initial_fields(cfg)
+    phi, pi = f0["phi"].copy(), f0["pi"].copy()
+    hub = P.HubbleState(cfg.T0, D.MU, cfg.eta_phys / D.MU)
+    dt, dx = D.MU * cfg.dt_phys, D.MU * cfg.dx_phys
+    k = 0
+    for n in range(cfg.Nt):
+        while k + 1 < len(tables) and tables[k + 1][0] <= n:
+            k += 1
+        _, t, tmin, dinv = tables[k]
+        s = hub.step(dt)
+        dx3 = D.MU ** 2 * cfg.dx_phys ** 3
+        if integrator == "overdamped":
+            ns = math.sqrt(2.0 * s["T"] * dt * s["inv_a3"] / (s["eta_eff"] * dx3))
+            phi, _ = O.langevin(O.OVERDAMPED, phi, None, dt, dx, s["eta_eff"], D.MU,
+                                s["inv_a2"], t, tmin, dinv, noise_seed=O.noise_seed(n, cfg.seed),
+                                noise_scale=ns)
+        else:
+            ns = math.sqrt(s["T"] * s["inv_a3"] / dx3)
+            phi, pi = O.baoab_step(phi, pi, dt, dx, s["eta_eff"], D.MU, s["inv_a2"], t, tmin,
+                                   dinv, noise_seed=O.noise_seed(n, cfg.seed), noise_scale=ns)
+    assert np.linalg.norm(phi_g - phi) <= 1e-12 * np.linalg.norm(phi)
+    if integrator == "baoab":
+        assert np.linalg.norm(pi_g - pi) <= 1e-12 * np.linalg.norm(pi)
+    else:
+        assert np.array_equal(pi_g, f0["pi"])
