@@ -326,6 +326,18 @@ __device__ __forceinline__ int plaquette(R x0, R y0, int h0, R x1, R y1, int h1,
     return -m;
 }
 
+// One lane per (y, z) column, marching over a chunk of kWindCX x planes: the four corners
+// (j, k), (j+1, k), (j, k+1), (j+1, k+1) of plane x+1 loaded for one plane are plane x's
+// corners for the next, so each plane costs four new (phi1, phi2) pairs.  The string and
+// plaquette counts are reduced over the block and added with one pair of atomics per block.
+constexpr int kWindCX = 16;
+
+template <typename R>
+struct Corner {
+    R x, y;
+    int h;
+};
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
                                                  const R* __restrict__ p2, int8_t* plaq,
@@ -334,53 +346,74 @@ __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
 {
     const int k = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
+    const int i0 = blockIdx.z * kWindCX;
+    const int i1 = min(i0 + kWindCX, nx);
     const bool act = k < nz && j < ny;
-    int nstr = 0, nplq = 0;
+    unsigned long long nstr = 0, nplq = 0;
     if (act) {
-        const int ip = i + 1 < nx ? i + 1 : 0;
         const int jp = j + 1 < ny ? j + 1 : 0;
         const int kp = k + 1 < nz ? k + 1 : 0;
         const int64_t n = (int64_t)nx * ny * nz;
-        auto at = [&](int a, int b, int c) { return ((int64_t)a * ny + b) * nz + c; };
-        const int64_t s = at(i, j, k);
-        const int64_t c_ijk = s, c_ipjk = at(ip, j, k), c_ipjpk = at(ip, jp, k),
-                      c_ijpk = at(i, jp, k), c_ipjkp = at(ip, j, kp), c_ijkp = at(i, j, kp),
-                      c_ijpkp = at(i, jp, kp);
-        R x00 = p1[c_ijk], y00 = p2[c_ijk];
-        R x10 = p1[c_ipjk], y10 = p2[c_ipjk];
-        R x11 = p1[c_ipjpk], y11 = p2[c_ipjpk];
-        R x01 = p1[c_ijpk], y01 = p2[c_ijpk];
-        R x1k = p1[c_ipjkp], y1k = p2[c_ipjkp];
-        R x0k = p1[c_ijkp], y0k = p2[c_ijkp];
-        R xjk = p1[c_ijpkp], yjk = p2[c_ijpkp];
-        int h00 = half_plane(x00, y00), h10 = half_plane(x10, y10), h11 = half_plane(x11, y11),
-            h01 = half_plane(x01, y01), h1k = half_plane(x1k, y1k), h0k = half_plane(x0k, y0k),
-            hjk = half_plane(xjk, yjk);
-        // XY: (i,j,k)->(ip,j,k)->(ip,jp,k)->(i,jp,k)
-        int wxy = plaquette(x00, y00, h00, x10, y10, h10, x11, y11, h11, x01, y01, h01);
-        // XZ: (i,j,k)->(ip,j,k)->(ip,j,kp)->(i,j,kp)
-        int wxz = plaquette(x00, y00, h00, x10, y10, h10, x1k, y1k, h1k, x0k, y0k, h0k);
-        // YZ: (i,j,k)->(i,jp,k)->(i,jp,kp)->(i,j,kp)
-        int wyz = plaquette(x00, y00, h00, x01, y01, h01, xjk, yjk, hjk, x0k, y0k, h0k);
-        if (plaq) {
-            plaq[s] = (int8_t)wxy;
-            plaq[n + s] = (int8_t)wxz;
-            plaq[2 * n + s] = (int8_t)wyz;
+        const int64_t nyz = (int64_t)ny * nz;
+        const int64_t o00 = (int64_t)j * nz + k, o10 = (int64_t)jp * nz + k;
+        const int64_t o01 = (int64_t)j * nz + kp, o11 = (int64_t)jp * nz + kp;
+        auto corner = [&](int64_t s) {
+            Corner<R> c;
+            c.x = p1[s];
+            c.y = p2[s];
+            c.h = half_plane(c.x, c.y);
+            return c;
+        };
+        // plane i corners: a = (j, k), b = (j+1, k), c = (j, k+1), d = (j+1, k+1)
+        int64_t base = (int64_t)i0 * nyz;
+        Corner<R> a = corner(base + o00), b = corner(base + o10), c = corner(base + o01),
+                  d = corner(base + o11);
+        for (int i = i0; i < i1; ++i) {
+            const int ip = i + 1 < nx ? i + 1 : 0;
+            const int64_t bp = (int64_t)ip * nyz;
+            const Corner<R> A = corner(bp + o00), B = corner(bp + o10), Cc = corner(bp + o01),
+                            D = corner(bp + o11);
+            // XY: (i,j,k)->(ip,j,k)->(ip,jp,k)->(i,jp,k)
+            const int wxy = plaquette(a.x, a.y, a.h, A.x, A.y, A.h, B.x, B.y, B.h, b.x, b.y, b.h);
+            // XZ: (i,j,k)->(ip,j,k)->(ip,j,kp)->(i,j,kp)
+            const int wxz =
+                plaquette(a.x, a.y, a.h, A.x, A.y, A.h, Cc.x, Cc.y, Cc.h, c.x, c.y, c.h);
+            // YZ: (i,j,k)->(i,jp,k)->(i,jp,kp)->(i,j,kp)
+            const int wyz = plaquette(a.x, a.y, a.h, b.x, b.y, b.h, d.x, d.y, d.h, c.x, c.y, c.h);
+            const int64_t s = base + o00;
+            if (plaq) {
+                plaq[s] = (int8_t)wxy;
+                plaq[n + s] = (int8_t)wxz;
+                plaq[2 * n + s] = (int8_t)wyz;
+            }
+            const int w = wxy + wxz + wyz;
+            if (wsum) wsum[s] = (float)w;
+            nstr += w != 0;
+            nplq += (wxy != 0) + (wxz != 0) + (wyz != 0);
+            a = A;
+            b = B;
+            c = Cc;
+            d = D;
+            base = bp;
         }
-        int w = wxy + wxz + wyz;
-        if (wsum) wsum[s] = (float)w;
-        nstr = w != 0;
-        nplq = (wxy != 0) + (wxz != 0) + (wyz != 0);
     }
     if (counts) {
-        // wave reductions; every lane of the wave reaches this point
-        unsigned long long a = (unsigned long long)__popcll(__ballot(nstr != 0));
-        int t = nplq;
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
-        if ((threadIdx.x & 63) == 0) {
+        // wave sums, then the four waves of the block through LDS; every thread gets here
+        for (int off = 32; off > 0; off >>= 1) {
+            nstr += __shfl_xor(nstr, off);
+            nplq += __shfl_xor(nplq, off);
+        }
+        __shared__ unsigned long long red[2][4];
+        if (threadIdx.x == 0) {
+            red[0][threadIdx.y] = nstr;
+            red[1][threadIdx.y] = nplq;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            const unsigned long long a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            const unsigned long long t = red[1][0] + red[1][1] + red[1][2] + red[1][3];
             if (a) atomicAdd(&counts[0], a);
-            if (t) atomicAdd(&counts[1], (unsigned long long)t);
+            if (t) atomicAdd(&counts[1], t);
         }
     }
 }
@@ -392,7 +425,8 @@ static int winding(const R* phi1, const R* phi2, int8_t* plaq, float* wsum,
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
     TIP_REQUIRE(phi1 && phi2, "tip_winding: NULL field");
-    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nx);
+    dim3 grid((unsigned)((nz + 63) / 64), (unsigned)((ny + 3) / 4),
+              (unsigned)((nx + kWindCX - 1) / kWindCX));
     hipLaunchKernelGGL((k_winding<R>), grid, dim3(64, 4), 0, as_stream(stream), phi1, phi2, plaq,
                        wsum, counts, (int)nx, (int)ny, (int)nz);
     TIP_HIP(hipGetLastError());
