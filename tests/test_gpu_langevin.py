@@ -360,3 +360,46 @@ def test_lattice_integrator_geometries(complex_field):
         ref = out[(0, 0, 1, 2), op]
         for k in init:
             assert rel(o[k], ref[k]) < 1e-6, (geo, op, k)
+
+
+def test_row_tile_fallbacks_and_errors():
+    """A shape the row tile cannot take falls back to the flat tile under the default geometry
+    (complex nz > 512) and agrees with the oracle; requesting the row kernel for such a shape
+    (or rows per thread not dividing ny) raises; a ragged real shape runs the row tile."""
+    from tip.lattice import KERNEL_ROWS, Lattice
+    f, q, tab, prm = complex_setup((3, 5, 600), 17)
+    lat = Lattice((3, 5, 600), np.float32, complex_field=True)
+    try:
+        for k, v in zip(("phi1", "phi2", "pi1", "pi2"), (*f, *q)):
+            lat.upload(k, v.astype(np.float32))
+        lat.set_table(*tab)
+        lat.step_overdamped(2, dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta"], mu=prm["mu"],
+                            noise_scale=prm["ns"], seed=0, step0=0)
+        got = lat.download("phi1").astype(np.float64)
+        with pytest.raises(RuntimeError):
+            lat.set_geometry(2, 0, True, KERNEL_ROWS)  # nz = 600 > 512
+    finally:
+        lat.close()
+    a, b = (x.astype(np.float32).astype(np.float64) for x in f)
+    for n in range(2):
+        a, b, _, _ = O.langevin_complex(O.OVERDAMPED, a, b, None, None, prm["dt"], prm["dx"],
+                                        prm["eta"], prm["mu"], 1.0, *tab,
+                                        noise_seed=O.noise_seed(n), noise_scale=prm["ns"])
+    assert rel(got, a) < 2e-5
+    phi0, pi0, tab_r, prm_r = real_setup((6, 10, 64), 19)
+    lat = Lattice((6, 10, 64), np.float32)
+    try:
+        lat.upload("phi", phi0.astype(np.float32))
+        lat.upload("pi", pi0.astype(np.float32))
+        lat.set_table(*tab_r, odd=True)
+        with pytest.raises(RuntimeError):
+            lat.set_geometry(4, 0, True, KERNEL_ROWS)  # 4 rows per thread do not divide ny = 10
+        # the default geometry takes the row tile with one row per thread: the step runs
+        lat.step_baoab(2, dt=prm_r["dt"], dx=prm_r["dx"], gamma=prm_r["eta"], mu=prm_r["mu"],
+                       inv_a2=prm_r["inv_a2"], noise_scale=prm_r["ns"], seed=0, step0=0)
+        got_phi = lat.download("phi").astype(np.float64)
+    finally:
+        lat.close()
+    rp, _ = oracle_real("baoab", phi0.astype(np.float32).astype(np.float64),
+                        pi0.astype(np.float32).astype(np.float64), tab_r, prm_r, 2)
+    assert rel(got_phi, rp) < 2e-5
