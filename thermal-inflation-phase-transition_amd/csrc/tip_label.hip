@@ -5,9 +5,9 @@
 // (6-connectivity, no wrap-around) and number components 1..n in raster order of their first
 // site.
 //
-// Labelling is union-find in HBM (after Playne and Hawick's GPU union-find): every foreground
-// site starts as its own parent; one pass merges each site with its -x, -y and -z foreground
-// neighbours by atomicMin on the roots, so a component's root is its smallest site index (its
+// Labelling is union-find (after Playne and Hawick's GPU union-find): each block first joins
+// the sites of one 8 x 8 x 16 brick in LDS, then a global pass joins the pieces across brick
+// faces by atomicMin on the roots in HBM, so a component's root is its smallest site index (its
 // first site in raster order); a compression pass points every site at its root; a scan of
 // the root flags numbers the roots 1..n in raster order, which is scipy's numbering.
 //
@@ -61,20 +61,91 @@ __device__ void unite(int32_t* L, int32_t a, int32_t b)
     }
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_ccl_init(const R* __restrict__ v, double thr, int64_t n,
-                                                  int32_t* __restrict__ L)
+// Local phase: a block labels one brick of kBX x kBY x kBZ sites with union-find in LDS (the
+// brick's own -x, -y, -z edges), then writes every foreground site's brick-local root as a
+// global index.  The local order (x, y, z within the brick) is monotone in the global site
+// index, so a local root -- the smallest local index of its piece -- is also the piece's
+// smallest global index; the global merge then only has to join pieces across brick faces.
+constexpr int kBX = 8, kBY = 8, kBZ = 16, kBrick = kBX * kBY * kBZ;
+
+__device__ __forceinline__ int lds_parent(const int* lab, int x)
 {
-    // the comparison runs in the field's precision, as numpy compares a float32 array with
-    // a Python float
-    const R t = (R)thr;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        const R a = v[s] < (R)0 ? -v[s] : v[s];
-        L[s] = a > t ? (int32_t)s : -1;
+    return __hip_atomic_load(lab + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int lds_find(const int* lab, int x)
+{
+    int p = lds_parent(lab, x);
+    while (p != x) {
+        x = p;
+        p = lds_parent(lab, x);
+    }
+    return x;
+}
+
+__device__ void lds_unite(int* lab, int a, int b)
+{
+    for (;;) {
+        a = lds_find(lab, a);
+        b = lds_find(lab, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicMin(lab + b, a);
+        if (old == b) return;
+        b = old;
     }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_ccl_local(const R* __restrict__ v, double thr,
+                                                   int32_t* __restrict__ L, int nx, int ny,
+                                                   int nz)
+{
+    __shared__ int lab[kBrick];
+    const int nbz = (nz + kBZ - 1) / kBZ, nby = (ny + kBY - 1) / kBY;
+    const int b = (int)blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int x0 = bx * kBX, y0 = by * kBY, z0 = bz * kBZ;
+    const R t = (R)thr;
+    auto gidx = [&](int li) {
+        const int lz = li % kBZ, ly = (li / kBZ) % kBY, lx = li / (kBZ * kBY);
+        return ((int64_t)(x0 + lx) * ny + (y0 + ly)) * nz + (z0 + lz);
+    };
+    auto inside = [&](int li) {
+        const int lz = li % kBZ, ly = (li / kBZ) % kBY, lx = li / (kBZ * kBY);
+        return x0 + lx < nx && y0 + ly < ny && z0 + lz < nz;
+    };
+    for (int li = threadIdx.x; li < kBrick; li += blockDim.x) {
+        int val = -1;
+        if (inside(li)) {
+            const R u = v[gidx(li)];
+            const R a = u < (R)0 ? -u : u;
+            if (a > t) val = li;
+        }
+        lab[li] = val;
+    }
+    __syncthreads();
+    for (int li = threadIdx.x; li < kBrick; li += blockDim.x) {
+        if (lab[li] < 0) continue;
+        const int lz = li % kBZ, ly = (li / kBZ) % kBY, lx = li / (kBZ * kBY);
+        if (lz > 0 && lab[li - 1] >= 0) lds_unite(lab, li, li - 1);
+        if (ly > 0 && lab[li - kBZ] >= 0) lds_unite(lab, li, li - kBZ);
+        if (lx > 0 && lab[li - kBZ * kBY] >= 0) lds_unite(lab, li, li - kBZ * kBY);
+    }
+    __syncthreads();
+    for (int li = threadIdx.x; li < kBrick; li += blockDim.x) {
+        if (!inside(li)) continue;
+        const int r = lab[li] >= 0 ? lds_find(lab, li) : -1;
+        L[gidx(li)] = r >= 0 ? (int32_t)gidx(r) : -1;
+    }
+}
+
+// Global phase: join the bricks' pieces across brick faces (and across the lattice faces when
+// periodic); edges inside a brick were joined in LDS.
 __global__ __launch_bounds__(256) void k_ccl_merge(int32_t* L, int nx, int ny, int nz,
                                                    int periodic)
 {
@@ -88,17 +159,17 @@ __global__ __launch_bounds__(256) void k_ccl_merge(int32_t* L, int nx, int ny, i
         const int i = (int)(r / ny);
         const int64_t plane = (int64_t)ny * nz;
         if (k > 0) {
-            if (L[s - 1] >= 0) unite(L, (int32_t)s, (int32_t)(s - 1));
+            if (k % kBZ == 0 && L[s - 1] >= 0) unite(L, (int32_t)s, (int32_t)(s - 1));
         } else if (periodic && nz > 1 && L[s + nz - 1] >= 0) {
             unite(L, (int32_t)s, (int32_t)(s + nz - 1));
         }
         if (j > 0) {
-            if (L[s - nz] >= 0) unite(L, (int32_t)s, (int32_t)(s - nz));
+            if (j % kBY == 0 && L[s - nz] >= 0) unite(L, (int32_t)s, (int32_t)(s - nz));
         } else if (periodic && ny > 1 && L[s + (int64_t)(ny - 1) * nz] >= 0) {
             unite(L, (int32_t)s, (int32_t)(s + (int64_t)(ny - 1) * nz));
         }
         if (i > 0) {
-            if (L[s - plane] >= 0) unite(L, (int32_t)s, (int32_t)(s - plane));
+            if (i % kBX == 0 && L[s - plane] >= 0) unite(L, (int32_t)s, (int32_t)(s - plane));
         } else if (periodic && nx > 1 && L[s + (int64_t)(nx - 1) * plane] >= 0) {
             unite(L, (int32_t)s, (int32_t)(s + (int64_t)(nx - 1) * plane));
         }
@@ -336,7 +407,11 @@ int label(const R* v, double thr, int64_t nx, int64_t ny, int64_t nz, int period
     int64_t* total = reinterpret_cast<int64_t*>(aux);
     int32_t* tiles = reinterpret_cast<int32_t*>(aux + 64);
     const unsigned g = grid_for(n);
-    hipLaunchKernelGGL((k_ccl_init<R>), dim3(g), dim3(256), 0, st, v, thr, n, labels);
+    const int64_t nbricks = ((nx + kBX - 1) / kBX) * ((ny + kBY - 1) / kBY) *
+                            ((nz + kBZ - 1) / kBZ);
+    TIP_REQUIRE(nbricks < (int64_t(1) << 31), "tip_label: too many bricks");
+    hipLaunchKernelGGL((k_ccl_local<R>), dim3((unsigned)nbricks), dim3(256), 0, st, v, thr,
+                       labels, (int)nx, (int)ny, (int)nz);
     hipLaunchKernelGGL(k_ccl_merge, dim3(g), dim3(256), 0, st, labels, (int)nx, (int)ny, (int)nz,
                        periodic ? 1 : 0);
     hipLaunchKernelGGL(k_ccl_compress, dim3(g), dim3(256), 0, st, labels, n);
