@@ -134,9 +134,9 @@ def ensemble_workload(npts: int = 8192, seed: int = 2024):
     """64 realisations: 32 x 64^3, 24 x 96^3, 8 x 128^3; seeds 1..64; T0 ~ U[6770, 7350] GeV;
     per-instance 8192-point float64 V' tables on [0, 1.5 phi_true(T0)] (odd)."""
     from tip import potential as P
-    from tip.ensemble import config3_shapes
+    from tip.ensemble import nucleation_ensemble_shapes
 
-    shapes = config3_shapes()
+    shapes = nucleation_ensemble_shapes()
     rng = np.random.default_rng(seed)
     T0s = rng.uniform(6770.0, 7350.0, size=len(shapes))
     p = P.ModelParams.real()
@@ -196,9 +196,9 @@ def bench_ensemble(steps: int = 200):
 
 
 def bench_extras(steps: int = 200) -> dict:
-    """The other forms on one GPU: complex 256^3 float32 step and winding (config 2 shape),
+    """The other forms on one GPU: complex 256^3 float32 step and winding,
     the float64 drop-in real 256^3 step, and the complex 512^3 step with the inline MLP
-    surrogate (config 4 shape; throughput does not depend on the weights, untrained ones are
+    surrogate (the 7350 -> 6770 K cooling workload's lattice; throughput does not depend on the weights, untrained ones are
     used)."""
     from tip import potential as P
     from tip import surrogate as S
@@ -386,7 +386,7 @@ def main() -> int:
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the "
-              "multi-GPU sharding is described in DESIGN.md, not implemented.", file=sys.stderr)
+              "multi-GPU sharding is not implemented.", file=sys.stderr)
         return 2
 
     import tip

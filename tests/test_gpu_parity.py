@@ -25,8 +25,8 @@ def _tip():
     return tip, kernels, kernels_complex, DeviceArray, Counter
 
 
-def c0_table(npts=8192):
-    """Config table: 8192 float64 points of V'(phi, T0=7000) on [0, 1.5 phi_true], odd."""
+def real_table(npts=8192):
+    """8192 float64 points of V'(phi, T0 = 7000) on [0, 1.5 phi_true], odd."""
     from tip import potential as P
     p = P.ModelParams.real()
     vt = P.thermal_vev(7000.0, p)
@@ -34,7 +34,7 @@ def c0_table(npts=8192):
     return t, tmin, dinv
 
 
-def c0_params():
+def real_params():
     from tip import potential as P
     mu, dx_phys, dt_phys, T0 = 1000.0, 0.002, 2.5e-4, 7000.0
     dt, dx, eta = mu * dt_phys, mu * dx_phys, T0 / mu
@@ -96,13 +96,13 @@ def _run_gpu_real(phi0, pi0, nsteps, prm, table, tmin, dinv, odd, dtype, seed=0)
 
 
 def test_f64_100_steps_c0_within_1e12():
-    """Config-0 shape (64^3 real), float64, 100 steps with hash noise: 1e-12 relative."""
+    """64^3 real lattice, float64, 100 steps with hash noise: 1e-12 relative."""
     rng = np.random.default_rng(2024)
     shape = (64, 64, 64)
     phi0 = 0.01 * rng.normal(size=shape)
     pi0 = np.zeros(shape)
-    prm = c0_params()
-    t, tmin, dinv = c0_table()
+    prm = real_params()
+    t, tmin, dinv = real_table()
     print(f"[f64 c0] shape={shape} ns={prm['noise_scale']:.4g}", flush=True)
     a, b = O.run_real(phi0, pi0, 100, prm, t, tmin, dinv, odd=True)
     g, h = _run_gpu_real(phi0, pi0, 100, prm, t, tmin, dinv, True, np.float64)
@@ -140,21 +140,21 @@ def test_f64_hash_noise_ragged_shapes(shape):
     rng = np.random.default_rng(sum(shape) + 1)
     phi0 = rng.normal(size=shape) * 3e3
     pi0 = rng.normal(size=shape) * 1e3
-    t, tmin, dinv = c0_table(1024)
-    prm = c0_params()
+    t, tmin, dinv = real_table(1024)
+    prm = real_params()
     a, b = O.run_real(phi0, pi0, 3, prm, t, tmin, dinv, odd=True, seed=9)
     g, h = _run_gpu_real(phi0, pi0, 3, prm, t, tmin, dinv, True, np.float64, seed=9)
     assert rel(g, a) <= 1e-12 and rel(h, b) <= 1e-12
 
 
 def test_f32_error_against_f64():
-    """float32 kernel vs float64 oracle after 100 steps (C0 shape): stated bound 1e-4."""
+    """float32 kernel vs float64 oracle after 100 steps (64^3 real lattice): stated bound 1e-4."""
     rng = np.random.default_rng(7)
     shape = (64, 64, 64)
     phi0 = 0.01 * rng.normal(size=shape)
     pi0 = np.zeros(shape)
-    prm = c0_params()
-    t, tmin, dinv = c0_table()
+    prm = real_params()
+    t, tmin, dinv = real_table()
     a, b = O.run_real(phi0, pi0, 100, prm, t, tmin, dinv, odd=True)
     g, h = _run_gpu_real(phi0, pi0, 100, prm, t, tmin, dinv, True, np.float32)
     e_phi, e_pi = rel(g.astype(np.float64), a), rel(h.astype(np.float64), b)
@@ -163,14 +163,14 @@ def test_f32_error_against_f64():
 
 
 def test_f32_long_run_against_f64_ensemble():
-    """Config-0 shape over 10^4 steps: the float32 path against the float64 path (itself within
+    """64^3 real lattice over 10^4 steps: the float32 path against the float64 path (itself within
     1e-12 of the oracle) for 4 seeds.  Past the first few hundred steps the fields of a chaotic
     Langevin run decorrelate whatever the precision, so the bar is on the true-vacuum fraction
     series: ensemble means within the ensembles' combined 1 sigma (plus one site)."""
     from tip.lattice import Lattice
     shape = (64, 64, 64)
-    prm = c0_params()
-    t, tmin, dinv = c0_table()
+    prm = real_params()
+    t, tmin, dinv = real_table()
     kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
               noise_scale=prm["noise_scale"])
     marks = list(range(100, 2001, 100)) + list(range(3000, 10001, 1000))
@@ -406,8 +406,8 @@ def test_lattice_geometries_bitwise_identical():
     shape = (40, 36, 96)
     phi0 = (rng.normal(size=shape) * 1e3).astype(np.float32)
     pi0 = (rng.normal(size=shape) * 1e2).astype(np.float32)
-    t, tmin, dinv = c0_table(1024)
-    prm = c0_params()
+    t, tmin, dinv = real_table(1024)
+    prm = real_params()
     kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
               noise_scale=prm["noise_scale"])
     ref = None

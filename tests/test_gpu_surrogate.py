@@ -4,7 +4,7 @@ Bars:
   * device forward pass vs the float64 oracle network: 2e-6 relative to the output scale
   * device gradients vs the oracle's analytic gradients (which finite differences pin):
     3e-4 of the largest component
-  * GPU Adam training (the config-4 recipe) lowers the loss > 1e4 x and meets
+  * GPU Adam training (the cooling-run recipe) lowers the loss > 1e4 x and meets
     |dV'|/max(|V'|, floor) <= 1e-4 against the float64 potential
   * V' assembly, on-device tabulation and the inline-MLP step agree with the oracle's use of
     the same network
@@ -27,7 +27,7 @@ def _spec():
 
 @pytest.fixture(scope="module")
 def trained():
-    """The config-4 training recipe: 2^20 float64-truth samples, weighted loss, 400 epochs of
+    """The cooling-run training recipe: 2^20 float64-truth samples, weighted loss, 400 epochs of
     GPU Adam (batch 1024, cosine lr 2e-3 -> 1e-6)."""
     from tip import surrogate as S
     spec = _spec()

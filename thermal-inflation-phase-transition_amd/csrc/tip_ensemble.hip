@@ -1,7 +1,7 @@
 // tip_ensemble.hip -- many independent real-field realisations (uneven lattice sizes, seeds,
 // temperatures and V' tables) advanced together.
 //
-// The reference runs one realisation per process (SURVEY: no ensemble layer exists); the
+// The reference runs one realisation per process (it has no ensemble layer); the
 // nucleation-rate and string statistics need ensembles.  Here every instance's float32 fields
 // live in one HBM pool, and one kernel launch per RK2 step covers all instances: the host
 // builds a work list of (instance, plane tile, x chunk) items and each block runs the same

@@ -339,12 +339,12 @@ class Run:
 
 def nucleation_ensemble(steps: int = 20000, report: int = 500, esc: float = 1e4,
                         frac: float = 0.5, mode: str = "graph", log=None):
-    """Config-3 run: 64 realisations (32 x 64^3, 24 x 96^3, 8 x 128^3), seeds 1..64,
+    """The nucleation ensemble: 64 realisations (32 x 64^3, 24 x 96^3, 8 x 128^3), seeds 1..64,
     T0 ~ U[6770, 7350], per-instance 8192-point tables; returns each instance's nucleation step
     (first report step with true-vacuum fraction > frac; -1 if none) and the fraction series."""
-    from .ensemble import Ensemble, config3_shapes
+    from .ensemble import Ensemble, nucleation_ensemble_shapes
 
-    shapes = config3_shapes()
+    shapes = nucleation_ensemble_shapes()
     rng = np.random.default_rng(2024)
     T0s = rng.uniform(6770.0, 7350.0, size=len(shapes))
     p = P.ModelParams.real()

@@ -97,6 +97,6 @@ class Ensemble:
             pass
 
 
-def config3_shapes():
+def nucleation_ensemble_shapes():
     """64 realisations of uneven size: 32 x 64^3, 24 x 96^3, 8 x 128^3 (46,399,488 sites)."""
     return [(64,) * 3] * 32 + [(96,) * 3] * 24 + [(128,) * 3] * 8

@@ -1,5 +1,6 @@
-"""Config-level experiments on top of the driver: the ensemble nucleation run (config 3) and
-the surrogate-versus-table comparison under continuous cooling (config 4).
+"""Experiments on top of the driver: the ensemble nucleation run (64 realisations of
+64^3-128^3) and the surrogate-versus-table comparison under continuous cooling (7350 -> 6770 K
+over 10^5 steps).
 
   python -m tip.experiments surrogate-vs-table --n 128 --steps 3000 --seeds 6 --report 100 \
          --cool-steps 100000 --exact-table
@@ -23,7 +24,7 @@ def surrogate_vs_table(surrogate, n: int = 64, steps: int = 4000, T0: float = 73
                        report: int = 250, esc: float = 1e4, cool_steps: int = None,
                        methods: Sequence[str] = ("table", "surrogate"), log=None) -> Dict:
     """Complex field, linear cooling T0 -> T1 over `cool_steps` steps (default `steps`; the
-    config-4 schedule is 10^5 steps, of which the first `steps` are run), an ensemble of seeds
+    full cooling run is 10^5 steps, of which the first `steps` are run), an ensemble of seeds
     run with each method: "table" -- the reference's table method (host-built 16384-point
     table of the reference's V' on the field range, rebuilt when |dT|/T > 1e-4 or the field
     leaves the range); "surrogate" -- the inline network at the exact T(t); "table_exact"
@@ -86,14 +87,14 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="what", required=True)
     a1 = sub.add_parser("surrogate-vs-table")
     a1.add_argument("--surrogate", default=None,
-                    help="trained npz (default: the shipped config-4 weights)")
+                    help="trained npz (default: the shipped cooling-run weights)")
     a1.add_argument("--train", action="store_true", help="train a fresh network first")
     a1.add_argument("--n", type=int, default=64)
     a1.add_argument("--steps", type=int, default=4000)
     a1.add_argument("--seeds", type=int, default=4)
     a1.add_argument("--report", type=int, default=250)
     a1.add_argument("--cool-steps", type=int, default=None,
-                    help="length of the cooling schedule (config 4: 100000)")
+                    help="length of the cooling schedule (the full run: 100000)")
     a1.add_argument("--exact-table", action="store_true",
                     help="also run the table method on the float64 potential")
     a2 = sub.add_parser("nucleation")
@@ -103,9 +104,9 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if a.what == "surrogate-vs-table":
-        from .surrogate import Surrogate, train_config4
+        from .surrogate import Surrogate, train_cooling_surrogate
         sur = (Surrogate.load(a.surrogate) if a.surrogate
-               else train_config4(log=log)[0] if a.train else Surrogate.load())
+               else train_cooling_surrogate(log=log)[0] if a.train else Surrogate.load())
         res = surrogate_vs_table(sur, n=a.n, steps=a.steps, seeds=range(a.seeds),
                                  report=a.report, cool_steps=a.cool_steps,
                                  methods=("table", "surrogate") + (("table_exact",)

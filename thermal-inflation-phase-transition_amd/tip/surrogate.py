@@ -228,8 +228,8 @@ class Surrogate:
 
     @classmethod
     def load(cls, path=None) -> "Surrogate":
-        """A saved network (default: the shipped config-4 weights)."""
-        spec, params = load_params_npz(path or TRAINED_CONFIG4)
+        """A saved network (default: the shipped weights for the 6500-7500 K cooling runs)."""
+        spec, params = load_params_npz(path or TRAINED_COOLING)
         return cls(spec, params=params)
 
     def error_report(self, n: int = 1 << 16, seed: int = 99) -> Dict[str, float]:
@@ -250,10 +250,10 @@ class Surrogate:
                 "n": int(n)}
 
 
-# Weights trained on an MI355X by train_config4(seed=0) (provenance and the error report at
+# Weights trained on an MI355X by train_cooling_surrogate(seed=0) (provenance and the error report at
 # training time are stored in the file's "meta" entry).
-TRAINED_CONFIG4 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
-                               "surrogate_config4.npz")
+TRAINED_COOLING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "surrogate_cooling_6500_7500.npz")
 
 
 def load_params_npz(path: str):
@@ -267,8 +267,8 @@ def load_params_npz(path: str):
     return spec, np.asarray(d["params"], np.float32)
 
 
-def train_config4(epochs: int = 400, n_samples: int = 1 << 20, seed: int = 0, log=None):
-    """The config-4 recipe: complex-field potential, T in [6500, 7500], 2^20 float64-truth
+def train_cooling_surrogate(epochs: int = 400, n_samples: int = 1 << 20, seed: int = 0, log=None):
+    """The recipe for the 7350 -> 6770 K cooling runs: complex-field potential, T in [6500, 7500], 2^20 float64-truth
     samples (5 % T margin), weighted loss, GPU Adam 400 epochs, batch 1024, cosine lr
     2e-3 -> 1e-6."""
     spec = SurrogateSpec()
