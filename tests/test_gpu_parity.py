@@ -95,7 +95,7 @@ def _run_gpu_real(phi0, pi0, nsteps, prm, table, tmin, dinv, odd, dtype, seed=0)
     return phi.download(), pi.download()
 
 
-def test_f64_100_steps_c0_within_1e12():
+def test_f64_100_steps_64cube_within_1e12():
     """64^3 real lattice, float64, 100 steps with hash noise: 1e-12 relative."""
     rng = np.random.default_rng(2024)
     shape = (64, 64, 64)
