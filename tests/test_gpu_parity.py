@@ -103,7 +103,7 @@ def test_f64_100_steps_64cube_within_1e12():
     pi0 = np.zeros(shape)
     prm = real_params()
     t, tmin, dinv = real_table()
-    print(f"[f64 c0] shape={shape} ns={prm['noise_scale']:.4g}", flush=True)
+    print(f"[f64 64^3] shape={shape} ns={prm['noise_scale']:.4g}", flush=True)
     a, b = O.run_real(phi0, pi0, 100, prm, t, tmin, dinv, odd=True)
     g, h = _run_gpu_real(phi0, pi0, 100, prm, t, tmin, dinv, True, np.float64)
     e_phi, e_pi = rel(g, a), rel(h, b)
