@@ -8,7 +8,7 @@ A massive free field (V' = c phi, linear table) at fixed T is run to equilibrium
 field is linear, the stationary covariance of every Fourier mode under each discrete scheme
 is known exactly (a 2 x 2 Lyapunov equation per mode: C = M C M^T + b b^T with the scheme's
 one-step matrix M and noise vector b; the overdamped scheme is scalar), so the measured
-variance is also held to that exact value within 1 % (measured: 0.07 %) -- a check of the noise amplitude, its
+variance is also held to that exact value within 1 % -- a check of the noise amplitude, its
 placement inside the step and the scheme's own damping, not only of the continuum limit.
 """
 import math
@@ -144,7 +144,8 @@ def test_kinetic_equipartition(scheme):
         if T == 7350.0:
             # white velocity spectrum: shell averages of |pi_k|^2 over the sub-Nyquist band
             kx = np.fft.fftfreq(N) * 2 * np.pi / dx
-            kk = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kx[None, None, :] ** 2)
+            kk = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2
+                         + kx[None, None, :] ** 2)
             k_ir, k_nyq = 2 * np.pi / (N * dx), np.pi / dx
             edges = np.linspace(4 * k_ir, 0.95 * k_nyq, 16)
             idx = np.digitize(kk.ravel(), edges)
