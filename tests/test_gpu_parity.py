@@ -396,6 +396,37 @@ def test_vacuum_counts_exact():
     assert K.vacuum_count(a, 1e4, b) == O.vacuum_count(a, 1e4, b)
 
 
+@pytest.mark.parametrize("n", [1, 3, 4097, 1 << 22, (1 << 22) + 3, 9_000_001])
+def test_vacuum_counts_sizes_and_alignment(n):
+    """Counts of 16-byte vector loads plus the element tail, several chunks per thread at the
+    largest sizes, and fields that start off a 16-byte boundary (element loads)."""
+    tip, K, KC, DA, Counter = _tip()
+    from tip import _lib
+    rng = np.random.default_rng(n)
+    for dtype, name in ((np.float32, "tip_vacuum_count_f32"), (np.float64, "tip_vacuum_count_f64")):
+        phi = (rng.normal(size=n + 1) * 1e4).astype(dtype)
+        d = DA.from_numpy(phi)
+        try:
+            for off in (0, 1):
+                c = Counter(1)
+                _lib.call(name, d.ptr + off * phi.itemsize, n, 1e4, c.ptr, None)
+                assert int(c.read()[0]) == O.vacuum_count(phi[off:off + n], 1e4), (dtype, off)
+        finally:
+            d.free()
+    a = (rng.normal(size=n + 1) * 1e4).astype(np.float32)
+    b = (rng.normal(size=n + 1) * 1e4).astype(np.float32)
+    da, db = DA.from_numpy(a), DA.from_numpy(b)
+    try:
+        for oa, ob in ((0, 0), (1, 1), (0, 1)):
+            c = Counter(1)
+            _lib.call("tip_vacuum_count_c32", da.ptr + 4 * oa, db.ptr + 4 * ob, n, 1e4, c.ptr,
+                      None)
+            assert int(c.read()[0]) == O.vacuum_count(a[oa:oa + n], 1e4, b[ob:ob + n])
+    finally:
+        da.free()
+        db.free()
+
+
 # ----------------------------------------------------------------------------------------
 # device-resident lattice: geometry independence and full-size properties
 # ----------------------------------------------------------------------------------------

@@ -285,15 +285,12 @@ extern "C" int tip_noise_field_f32(float* noise1, float* noise2, int64_t nx, int
 template <typename R>
 __device__ __forceinline__ int half_plane(R x, R y)
 {
-    // sign of atan2(y, x) as the reference's atan2 returns it, including signed zeros.
-    if (y > (R)0) return 1;
-    if (y < (R)0) return -1;
-    if (y == (R)0) {
-        bool left = (x < (R)0) || (x == (R)0 && signbit(x));
-        if (!left) return 0;
-        return signbit(y) ? -1 : 1;
-    }
-    return 0;  // NaN
+    // sign of atan2(y, x) as the reference's atan2 returns it, including signed zeros:
+    // +-1 for y != 0, and for y = +-0 the sign of y when x is on the negative side (x < 0 or
+    // x = -0), else 0; NaN gives 0.  Selects only, no branches.
+    const bool left = signbit(x) && !(x != x);
+    const bool on = (y != (R)0 || left) && !(y != y);
+    return on ? (signbit(y) ? -1 : 1) : 0;
 }
 
 // cross(a, b) == 0 with a and b in opposite open half planes: a zero vector whose signed zeros
@@ -307,36 +304,52 @@ __device__ __noinline__ int edge_wrap_degenerate(double ax, double ay, double bx
     return (int)rint((d - w) * 0.15915494309189535);  // 1 / (2 pi)
 }
 
-template <typename R>
-__device__ __forceinline__ int edge_wrap(R ax, R ay, int ha, R bx, R by, int hb)
-{
-    if (ha == hb || ha == 0 || hb == 0) return 0;
-    double cr = (double)ax * (double)by - (double)ay * (double)bx;
-    if (cr == 0.0) return edge_wrap_degenerate((double)ax, (double)ay, (double)bx, (double)by);
-    if (ha < 0) return cr < 0.0 ? 1 : 0;
-    return cr > 0.0 ? -1 : 0;
-}
-
-template <typename R>
-__device__ __forceinline__ int plaquette(R x0, R y0, int h0, R x1, R y1, int h1, R x2, R y2, int h2,
-                                         R x3, R y3, int h3)
-{
-    int m = edge_wrap(x0, y0, h0, x1, y1, h1) + edge_wrap(x1, y1, h1, x2, y2, h2) +
-            edge_wrap(x2, y2, h2, x3, y3, h3) + edge_wrap(x3, y3, h3, x0, y0, h0);
-    return -m;
-}
-
-// One lane per (y, z) column, marching over a chunk of kWindCX x planes: the four corners
-// (j, k), (j+1, k), (j, k+1), (j+1, k+1) of plane x+1 loaded for one plane are plane x's
-// corners for the next, so each plane costs four new (phi1, phi2) pairs.  The string and
-// plaquette counts are reduced over the block and added with one pair of atomics per block.
-constexpr int kWindCX = 16;
-
-template <typename R>
-struct Corner {
-    R x, y;
+// A plaquette corner: the field in float64 (exact for float32 input) and its half plane h.
+struct WCorner {
+    double x, y;
     int h;
 };
+
+template <typename R>
+__device__ __forceinline__ WCorner wcorner(R x, R y)
+{
+    WCorner c;
+    c.x = (double)x;
+    c.y = (double)y;
+    c.h = half_plane(x, y);
+    return c;
+}
+
+// Wrap count m of the edge a -> b: +1 for h_a = -1, h_b = +1 and cross < 0; -1 for h_a = +1,
+// h_b = -1 and cross > 0; else 0.  With t = h_b - h_a and s = -sign(cross) this is
+// m = (t == 2 s) ? s : 0 (t = +-2 exactly when the corners are in opposite half planes).
+// `deg` is raised where they are and cross is exactly zero (edge_m_full decides those).
+// m(b -> a) = -m(a -> b) in both the sign method and the degenerate expression (the
+// difference d changes sign exactly, sin and atan2 are odd, rint is odd), so each edge is
+// evaluated once and enters its plaquettes with the orientation's sign.
+__device__ __forceinline__ int edge_m(const WCorner& a, const WCorner& b, bool& deg)
+{
+    const double cr = a.x * b.y - a.y * b.x;
+    const int t = b.h - a.h;
+    const int s = (int)(cr < 0.0) - (int)(cr > 0.0);
+    deg |= (t * t == 4) & (cr == 0.0);
+    return t == 2 * s ? s : 0;
+}
+
+__device__ __forceinline__ int edge_m_full(const WCorner& a, const WCorner& b)
+{
+    bool deg = false;
+    const int m = edge_m(a, b, deg);
+    return deg ? edge_wrap_degenerate(a.x, a.y, b.x, b.y) : m;
+}
+
+// One lane per (y, z) column, marching over a chunk of kWindCX x planes.  Plane x's corners
+// (j, k), (j+1, k), (j, k+1), (j+1, k+1) are the previous plane's x+1 corners, and its y and
+// z edges are the previous plane's x+1 edges, so each plane evaluates seven new edges: three
+// along x, two along y and two along z at x+1.  The loads of plane x+2 are issued before
+// plane x is evaluated.  The string and plaquette counts are reduced over the block and added
+// with one pair of atomics per block.
+constexpr int kWindCX = 16;
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
@@ -349,7 +362,7 @@ __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
     const int i0 = blockIdx.z * kWindCX;
     const int i1 = min(i0 + kWindCX, nx);
     const bool act = k < nz && j < ny;
-    unsigned long long nstr = 0, nplq = 0;
+    unsigned nstr = 0, nplq = 0;
     if (act) {
         const int jp = j + 1 < ny ? j + 1 : 0;
         const int kp = k + 1 < nz ? k + 1 : 0;
@@ -357,29 +370,54 @@ __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
         const int64_t nyz = (int64_t)ny * nz;
         const int64_t o00 = (int64_t)j * nz + k, o10 = (int64_t)jp * nz + k;
         const int64_t o01 = (int64_t)j * nz + kp, o11 = (int64_t)jp * nz + kp;
-        auto corner = [&](int64_t s) {
-            Corner<R> c;
-            c.x = p1[s];
-            c.y = p2[s];
-            c.h = half_plane(c.x, c.y);
-            return c;
-        };
-        // plane i corners: a = (j, k), b = (j+1, k), c = (j, k+1), d = (j+1, k+1)
+        // corners of plane x: a = (j, k), b = (j+1, k), c = (j, k+1), d = (j+1, k+1)
         int64_t base = (int64_t)i0 * nyz;
-        Corner<R> a = corner(base + o00), b = corner(base + o10), c = corner(base + o01),
-                  d = corner(base + o11);
+        WCorner a = wcorner(p1[base + o00], p2[base + o00]);
+        WCorner b = wcorner(p1[base + o10], p2[base + o10]);
+        WCorner c = wcorner(p1[base + o01], p2[base + o01]);
+        WCorner d = wcorner(p1[base + o11], p2[base + o11]);
+        int ey = edge_m_full(a, b), ez = edge_m_full(a, c), ezb = edge_m_full(b, d),
+            eyc = edge_m_full(c, d);
+        // raw corners of plane x+1
+        int64_t bn = (int64_t)(i0 + 1 < nx ? i0 + 1 : 0) * nyz;
+        R q[8] = {p1[bn + o00], p2[bn + o00], p1[bn + o10], p2[bn + o10],
+                  p1[bn + o01], p2[bn + o01], p1[bn + o11], p2[bn + o11]};
         for (int i = i0; i < i1; ++i) {
-            const int ip = i + 1 < nx ? i + 1 : 0;
-            const int64_t bp = (int64_t)ip * nyz;
-            const Corner<R> A = corner(bp + o00), B = corner(bp + o10), Cc = corner(bp + o01),
-                            D = corner(bp + o11);
-            // XY: (i,j,k)->(ip,j,k)->(ip,jp,k)->(i,jp,k)
-            const int wxy = plaquette(a.x, a.y, a.h, A.x, A.y, A.h, B.x, B.y, B.h, b.x, b.y, b.h);
+            const WCorner A = wcorner(q[0], q[1]), B = wcorner(q[2], q[3]);
+            const WCorner Cc = wcorner(q[4], q[5]), D = wcorner(q[6], q[7]);
+            if (i + 1 < i1) {
+                int i2 = i + 2;
+                if (i2 >= nx) i2 -= nx;
+                if (i2 >= nx) i2 -= nx;
+                const int64_t b2 = (int64_t)i2 * nyz;
+                q[0] = p1[b2 + o00];
+                q[1] = p2[b2 + o00];
+                q[2] = p1[b2 + o10];
+                q[3] = p2[b2 + o10];
+                q[4] = p1[b2 + o01];
+                q[5] = p2[b2 + o01];
+                q[6] = p1[b2 + o11];
+                q[7] = p2[b2 + o11];
+            }
+            bool deg = false;
+            int exa = edge_m(a, A, deg), exb = edge_m(b, B, deg), exc = edge_m(c, Cc, deg);
+            int eyA = edge_m(A, B, deg), ezA = edge_m(A, Cc, deg), ezB = edge_m(B, D, deg);
+            int eyC = edge_m(Cc, D, deg);
+            if (deg) {  // rare: some edge of this plane needs the transcendental expression
+                exa = edge_m_full(a, A);
+                exb = edge_m_full(b, B);
+                exc = edge_m_full(c, Cc);
+                eyA = edge_m_full(A, B);
+                ezA = edge_m_full(A, Cc);
+                ezB = edge_m_full(B, D);
+                eyC = edge_m_full(Cc, D);
+            }
+            // XY: (i,j,k)->(ip,j,k)->(ip,jp,k)->(i,jp,k); the winding is -(sum of m)
+            const int wxy = ey + exb - exa - eyA;
             // XZ: (i,j,k)->(ip,j,k)->(ip,j,kp)->(i,j,kp)
-            const int wxz =
-                plaquette(a.x, a.y, a.h, A.x, A.y, A.h, Cc.x, Cc.y, Cc.h, c.x, c.y, c.h);
+            const int wxz = ez + exc - exa - ezA;
             // YZ: (i,j,k)->(i,jp,k)->(i,jp,kp)->(i,j,kp)
-            const int wyz = plaquette(a.x, a.y, a.h, b.x, b.y, b.h, d.x, d.y, d.h, c.x, c.y, c.h);
+            const int wyz = ez + eyc - ey - ezb;
             const int64_t s = base + o00;
             if (plaq) {
                 plaq[s] = (int8_t)wxy;
@@ -394,7 +432,11 @@ __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
             b = B;
             c = Cc;
             d = D;
-            base = bp;
+            ey = eyA;
+            ez = ezA;
+            ezb = ezB;
+            eyc = eyC;
+            base = (int64_t)(i + 1 < nx ? i + 1 : 0) * nyz;
         }
     }
     if (counts) {
@@ -403,15 +445,17 @@ __global__ __launch_bounds__(256) void k_winding(const R* __restrict__ p1,
             nstr += __shfl_xor(nstr, off);
             nplq += __shfl_xor(nplq, off);
         }
-        __shared__ unsigned long long red[2][4];
+        __shared__ unsigned red[2][4];
         if (threadIdx.x == 0) {
             red[0][threadIdx.y] = nstr;
             red[1][threadIdx.y] = nplq;
         }
         __syncthreads();
         if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const unsigned long long a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-            const unsigned long long t = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+            const unsigned long long a = (unsigned long long)red[0][0] + red[0][1] + red[0][2] +
+                                         red[0][3];
+            const unsigned long long t = (unsigned long long)red[1][0] + red[1][1] + red[1][2] +
+                                         red[1][3];
             if (a) atomicAdd(&counts[0], a);
             if (t) atomicAdd(&counts[1], t);
         }
@@ -449,25 +493,113 @@ extern "C" int tip_winding_f64(const double* phi1, const double* phi2, int8_t* p
 
 // ------------------------------------------------------------------------------------------
 // Kernel (c): true-vacuum count, and min/max reductions.
+//
+// The count reads one or two fields once and does almost no arithmetic, so it is bound by the
+// bytes in flight: each thread issues kVacU independent 16-byte loads per field before testing
+// any of them, and the block's count is added with one atomic (at most kVacMaxBlocks atomics
+// per launch on one address).  Fields that are not 16-byte aligned take the same loop with
+// element loads.
 // ------------------------------------------------------------------------------------------
-template <typename R, bool CPLX>
-__global__ __launch_bounds__(256) void k_vacuum(const R* __restrict__ a, const R* __restrict__ b,
-                                                int64_t n, double esc,
-                                                unsigned long long* count)
+constexpr int kVacThreads = 1024;
+constexpr int kVacU = 4;
+constexpr int kVacMaxBlocks = 512;
+
+template <typename R, int L>
+struct Pack {
+    R e[L];
+};
+
+template <typename R, int L>
+__device__ __forceinline__ Pack<R, L> load_pack(const R* __restrict__ p, int64_t i)
 {
-    unsigned long long c = 0;
+    Pack<R, L> r;
+    if constexpr (L == 1) {
+        r.e[0] = p[i];
+    } else {
+        typedef R V __attribute__((ext_vector_type(L)));
+        const V v = reinterpret_cast<const V*>(p)[i];
+#pragma unroll
+        for (int l = 0; l < L; ++l) r.e[l] = v[l];
+    }
+    return r;
+}
+
+template <typename R, bool CPLX>
+__device__ __forceinline__ unsigned vac_hit(R u, R v, double esc, double e2)
+{
+    if (CPLX) {
+        const double a = (double)u, b = (double)v;
+        return (a * a + b * b > e2) ? 1u : 0u;
+    }
+    return (fabs((double)u) > esc) ? 1u : 0u;
+}
+
+template <typename R, bool CPLX, int L>
+__global__ __launch_bounds__(kVacThreads) void k_vacuum(const R* __restrict__ a,
+                                                        const R* __restrict__ b, int64_t n,
+                                                        double esc,
+                                                        unsigned long long* count)
+{
     const double e2 = esc * esc;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        if (CPLX) {
-            double u = (double)a[s], v = (double)b[s];
-            c += (u * u + v * v > e2) ? 1 : 0;
-        } else {
-            c += (fabs((double)a[s]) > esc) ? 1 : 0;
+    const int64_t nv = n / L;
+    const int64_t span = (int64_t)gridDim.x * kVacThreads * kVacU;
+    unsigned c = 0;  // per thread at most n / (kVacThreads * blocks) + kVacU * L + 1
+    for (int64_t s0 = (int64_t)blockIdx.x * kVacThreads * kVacU + threadIdx.x; s0 < nv;
+         s0 += span) {
+        Pack<R, L> x[kVacU], y[kVacU];
+#pragma unroll
+        for (int u = 0; u < kVacU; ++u) {
+            const int64_t s = s0 + (int64_t)u * kVacThreads;
+            x[u] = Pack<R, L>{};
+            y[u] = Pack<R, L>{};
+            if (s < nv) {
+                x[u] = load_pack<R, L>(a, s);
+                if (CPLX) y[u] = load_pack<R, L>(b, s);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kVacU; ++u) {
+            if (s0 + (int64_t)u * kVacThreads < nv) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) c += vac_hit<R, CPLX>(x[u].e[l], y[u].e[l], esc, e2);
+            }
         }
     }
+    // the n mod L elements past the last full vector
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - nv * L) {
+        const int64_t s = nv * L + threadIdx.x;
+        c += vac_hit<R, CPLX>(a[s], CPLX ? b[s] : (R)0, esc, e2);
+    }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+    __shared__ unsigned red[kVacThreads / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kVacThreads / 64; ++w) t += red[w];
+        if (t) atomicAdd(count, t);
+    }
+}
+
+template <typename R, bool CPLX>
+static int vacuum_count(const R* a, const R* b, int64_t n, double esc, unsigned long long* count,
+                        void* stream)
+{
+    if (n == 0) return TIP_OK;
+    constexpr int L = 16 / sizeof(R);
+    const bool vec = ((uintptr_t)a % 16 == 0) && (!CPLX || (uintptr_t)b % 16 == 0);
+    const int64_t nv = vec ? n / L : n;
+    int64_t nb = (nv + (int64_t)kVacThreads * kVacU - 1) / ((int64_t)kVacThreads * kVacU);
+    nb = nb < 1 ? 1 : (nb > kVacMaxBlocks ? kVacMaxBlocks : nb);
+    hipStream_t st = as_stream(stream);
+    if (vec)
+        hipLaunchKernelGGL((k_vacuum<R, CPLX, L>), dim3((unsigned)nb), dim3(kVacThreads), 0, st,
+                           a, b, n, esc, count);
+    else
+        hipLaunchKernelGGL((k_vacuum<R, CPLX, 1>), dim3((unsigned)nb), dim3(kVacThreads), 0, st,
+                           a, b, n, esc, count);
+    TIP_HIP(hipGetLastError());
+    return TIP_OK;
 }
 
 static unsigned reduce_blocks(int64_t n)
@@ -482,33 +614,21 @@ extern "C" int tip_vacuum_count_f32(const float* phi, int64_t n, double esc,
                                     unsigned long long* count, void* stream)
 {
     TIP_REQUIRE(phi && count && n >= 0, "tip_vacuum_count_f32: bad arguments");
-    if (n == 0) return TIP_OK;
-    hipLaunchKernelGGL((k_vacuum<float, false>), dim3(reduce_blocks(n)), dim3(256), 0,
-                       as_stream(stream), phi, nullptr, n, esc, count);
-    TIP_HIP(hipGetLastError());
-    return TIP_OK;
+    return vacuum_count<float, false>(phi, nullptr, n, esc, count, stream);
 }
 
 extern "C" int tip_vacuum_count_f64(const double* phi, int64_t n, double esc,
                                     unsigned long long* count, void* stream)
 {
     TIP_REQUIRE(phi && count && n >= 0, "tip_vacuum_count_f64: bad arguments");
-    if (n == 0) return TIP_OK;
-    hipLaunchKernelGGL((k_vacuum<double, false>), dim3(reduce_blocks(n)), dim3(256), 0,
-                       as_stream(stream), phi, nullptr, n, esc, count);
-    TIP_HIP(hipGetLastError());
-    return TIP_OK;
+    return vacuum_count<double, false>(phi, nullptr, n, esc, count, stream);
 }
 
 extern "C" int tip_vacuum_count_c32(const float* phi1, const float* phi2, int64_t n, double esc,
                                     unsigned long long* count, void* stream)
 {
     TIP_REQUIRE(phi1 && phi2 && count && n >= 0, "tip_vacuum_count_c32: bad arguments");
-    if (n == 0) return TIP_OK;
-    hipLaunchKernelGGL((k_vacuum<float, true>), dim3(reduce_blocks(n)), dim3(256), 0,
-                       as_stream(stream), phi1, phi2, n, esc, count);
-    TIP_HIP(hipGetLastError());
-    return TIP_OK;
+    return vacuum_count<float, true>(phi1, phi2, n, esc, count, stream);
 }
 
 // Two-stage min/max: per-block partials into scratch, then one block folds them.
