@@ -271,6 +271,12 @@ __device__ __forceinline__ void lv_rows_tile(
         return make_rsrc(reinterpret_cast<const char*>(b) + (size_t)x * plane, pb);
     };
     const __amdgpu_buffer_rsrc_t T = make_rsrc(tab.ab, (uint32_t)(tab.last + 1) * 8u);
+    TableRadial rad;  // complex tables are even (ODD = false); unused for the real field
+    rad.t.ab = tab.ab;
+    rad.t.tmin = tab.tmin;
+    rad.t.dinv = tab.dinv;
+    rad.t.off = tab.off;
+    rad.t.last = tab.last;
     uint32_t off[TY], offs[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
@@ -311,6 +317,18 @@ __device__ __forceinline__ void lv_rows_tile(
                           const float (&qc)[NF][TY], float (&qn)[NF][TY],
                           const float (&hc)[NF][2], float (&hn)[NF][2]) {
         constexpr int b = decltype(bufc)::value;
+        // Complex field: the radii and table gathers of every row depend only on this plane's
+        // phi; they are issued before the next plane's loads so that waiting for a gather
+        // (vector memory operations complete in issue order) does not wait for the next plane.
+        float rho[TY];
+        TableRadial::Pre seg[TY];
+        if constexpr (NF == 2) {
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                rho[r] = complex_rho(pc[0][r], pc[1][r]);
+                seg[r] = rad.fetch(rho[r]);
+            }
+        }
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -350,9 +368,8 @@ __device__ __forceinline__ void lv_rows_tile(
             if constexpr (NF == 1) {
                 dV[0] = seg_eval<ODD>(seg_load<ODD>(T, tab, pc[0][r]), pc[0][r]);
             } else {
-                TableRadial rad;
-                rad.t = tab;  // complex tables are even (ODD = false)
-                complex_force_f32(rad, zn, pc[0][r], pc[NF - 1][r], dV[0], dV[NF - 1]);
+                complex_force_finish(rad, zn, pc[0][r], pc[NF - 1][r], rho[r], seg[r], dV[0],
+                                     dV[NF - 1]);
             }
 #pragma unroll
             for (int f = 0; f < NF; ++f) {

@@ -760,13 +760,22 @@ __device__ __forceinline__ void gauss_pair_f32_l2(uint64_t seed, uint32_t word_m
 }
 
 // V'(rho) of the complex field's radial potential: the float32 segment table ...
+// fetch() issues the gather, eval() finishes V' from what it returned, so a caller can issue
+// the gathers of several sites before it needs any of them.
 struct TableRadial {
     TableF32<false> t;
-    __device__ __forceinline__ float operator()(float rho) const
+    using Pre = float2;
+    static constexpr bool kEarly = true;  // issue the gathers of a plane before its loads
+    __device__ __forceinline__ Pre fetch(float rho) const
     {
         const __amdgpu_buffer_rsrc_t T = make_rsrc(t.ab, (uint32_t)(t.last + 1) * 8u);
-        return seg_eval<false>(seg_load<false>(T, t, rho), rho);
+        return seg_load<false>(T, t, rho);
     }
+    __device__ __forceinline__ float eval(const Pre& s, float rho) const
+    {
+        return seg_eval<false>(s, rho);
+    }
+    __device__ __forceinline__ float operator()(float rho) const { return eval(fetch(rho), rho); }
 };
 
 // ... or the learned network at this step's temperature, read from its per-step grid in the
@@ -778,7 +787,13 @@ struct MlpGridRadial {
     int last;  // number of segments - 1
     float inv_rho_s, gscale, out_scale, inv_rho_cut;
     double lam, m2;
-    __device__ __forceinline__ float operator()(float rho) const
+    struct Pre {
+        float a, b, frac;  // segment (y_i, y_{i+1} - y_i) and the position inside it
+    };
+    // three registers per pending gather (and its log and sqrt) cost a wave per SIMD here:
+    // the gathers are issued where they are used
+    static constexpr bool kEarly = false;
+    __device__ __forceinline__ Pre fetch(float rho) const
     {
         const __amdgpu_buffer_rsrc_t Y = make_rsrc(seg, (uint32_t)(last + 1) * 8u);
         const float s = rho * inv_rho_s;
@@ -787,22 +802,34 @@ struct MlpGridRadial {
         using f2 = float __attribute__((ext_vector_type(2)));
         const f2 ab =
             __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(Y, idx * 8, 0, 0));
-        const float y = fmaf(ab.y, g - (float)idx, ab.x);
+        return Pre{ab.x, ab.y, g - (float)idx};
+    }
+    __device__ __forceinline__ float eval(const Pre& p, float rho) const
+    {
+        const float y = fmaf(p.b, p.frac, p.a);
         const float q = rho * inv_rho_cut;
         const float th = out_scale * y / fmaf(q, q, 1.0f);
         const double r = (double)rho;
         return (float)(r * (fma(lam * r, r, -m2) + (double)th));
     }
+    __device__ __forceinline__ float operator()(float rho) const { return eval(fetch(rho), rho); }
 };
 
-// dV_i = V'(rho) phi_i / max(rho, 1e-30) plus the Z_N term.
-template <typename Radial>
-__device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnParams<float>& zn,
-                                                  float a, float b, float& d1, float& d2)
+// dV_i = V'(rho) phi_i / max(rho, 1e-30) plus the Z_N term.  complex_rho + Radial::fetch
+// start it (the gather), complex_force_finish completes it.
+__device__ __forceinline__ float complex_rho(float a, float b)
 {
-    const float rho = __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
+    return __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
+}
+
+template <typename Radial>
+__device__ __forceinline__ void complex_force_finish(const Radial& rad, const ZnParams<float>& zn,
+                                                     float a, float b, float rho,
+                                                     const typename Radial::Pre& pre, float& d1,
+                                                     float& d2)
+{
     const float inv = __builtin_amdgcn_rcpf(fmaxf(rho, 1e-30f));
-    const float w = rad(rho) * inv;
+    const float w = rad.eval(pre, rho) * inv;
     d1 = w * a;
     d2 = w * b;
     if (zn.active && rho > 1e-20f) {
@@ -825,6 +852,14 @@ __device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnPar
     }
 }
 
+template <typename Radial>
+__device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnParams<float>& zn,
+                                                  float a, float b, float& d1, float& d2)
+{
+    const float rho = complex_rho(a, b);
+    complex_force_finish(rad, zn, a, b, rho, rad.fetch(rho), d1, d2);
+}
+
 template <int TY, int NM, typename Radial>
 __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
@@ -837,7 +872,8 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     constexpr int NF = 4;                 // phi1, phi2, pi1, pi2
     constexpr int K = 2 * NF * TY + 1;    // LDS words per z column
     const int G = (int)blockDim.y / W;
-    const int w = (int)threadIdx.y, grp = w / W, zw = w - grp * W;
+    // blockDim.x is one wave: threadIdx.y and the row offsets derived from it are wave-uniform
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y), grp = w / W, zw = w - grp * W;
     const int z = zw * 64 + (int)threadIdx.x;
     const bool zact = z < nz;
     const int zc = zact ? z : 0;
@@ -858,14 +894,15 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     char* O[NF] = {reinterpret_cast<char*>(o1), reinterpret_cast<char*>(o2),
                    reinterpret_cast<char*>(o3), reinterpret_cast<char*>(o4)};
     auto at = [&](const char* bp, int x) { return make_rsrc(bp + (size_t)x * plane, pb); };
-    uint32_t off[TY], offs[TY];
+    // z column offset per lane (vector register), row offsets wave-uniform (scalar registers);
+    // idle lanes store at the plane's end (< 2^31 bytes, no wrap), where stores are dropped
+    const uint32_t vo = (uint32_t)zc * 4u;
+    const uint32_t vs = act ? vo : pb;
+    uint32_t so[TY];
 #pragma unroll
-    for (int r = 0; r < TY; ++r) {
-        off[r] = (uint32_t)((yb + r) * nz + zc) * 4u;
-        offs[r] = act ? off[r] : 0xFFFFFFF0u;
-    }
-    const uint32_t offm = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz + zc) * 4u;
-    const uint32_t offp = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz + zc) * 4u;
+    for (int r = 0; r < TY; ++r) so[r] = (uint32_t)((yb + r) * nz) * 4u;
+    const uint32_t som = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz) * 4u;
+    const uint32_t sop = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz) * 4u;
     const uint64_t planeK = (uint64_t)(uint32_t)nyz * (uint64_t)kNoiseMult;
     const uint64_t col0K = (uint64_t)(uint32_t)(yb * nz + zc) * (uint64_t)kNoiseMult;
     const uint64_t rowK = (uint64_t)(uint32_t)nz * (uint64_t)kNoiseMult;
@@ -879,16 +916,16 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
-                v0[f][r] = buf_load<float>(at(F[f], xm), off[r]);
-                v1[f][r] = buf_load<float>(at(F[f], x0), off[r]);
-                v2[f][r] = buf_load<float>(at(F[f], xq), off[r]);
+                v0[f][r] = buf_load_s(at(F[f], xm), vo, so[r]);
+                v1[f][r] = buf_load_s(at(F[f], x0), vo, so[r]);
+                v2[f][r] = buf_load_s(at(F[f], xq), vo, so[r]);
             }
     }
     float hA[2 * NF], hB[2 * NF];  // [f] row y0-1, [NF + f] row y0+TY
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        hA[f] = buf_load<float>(at(F[f], x0), offm);
-        hA[NF + f] = buf_load<float>(at(F[f], x0), offp);
+        hA[f] = buf_load_s(at(F[f], x0), vo, som);
+        hA[NF + f] = buf_load_s(at(F[f], x0), vo, sop);
     }
     unsigned cnt = 0;
 
@@ -896,6 +933,21 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                           const float (&vp)[NF][TY], float (&vn)[NF][TY],
                           const float (&hc)[2 * NF], float (&hn)[2 * NF]) {
         constexpr int b = decltype(bufc)::value;
+        // Both stages' radii and table gathers of every row depend only on this plane's
+        // fields: they are issued before the next plane's loads, so waiting for a gather
+        // (vector memory operations complete in issue order) never waits for the next plane.
+        float rho1[TY], rho2[TY];
+        typename Radial::Pre P1[TY], P2[TY];
+        if constexpr (Radial::kEarly) {
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                rho1[r] = complex_rho(vc[0][r], vc[1][r]);
+                P1[r] = rad.fetch(rho1[r]);
+                rho2[r] = complex_rho(fmaf(c.half_dt, vc[2][r], vc[0][r]),
+                                      fmaf(c.half_dt, vc[3][r], vc[1][r]));
+                P2[r] = rad.fetch(rho2[r]);
+            }
+        }
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -906,9 +958,9 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
 #pragma unroll
-                for (int r = 0; r < TY; ++r) vn[f][r] = buf_load<float>(at(F[f], xb), off[r]);
-                hn[f] = buf_load<float>(at(F[f], xa), offm);
-                hn[NF + f] = buf_load<float>(at(F[f], xa), offp);
+                for (int r = 0; r < TY; ++r) vn[f][r] = buf_load_s(at(F[f], xb), vo, so[r]);
+                hn[f] = buf_load_s(at(F[f], xa), vo, som);
+                hn[NF + f] = buf_load_s(at(F[f], xa), vo, sop);
             }
         }
         __syncthreads();
@@ -927,22 +979,28 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             const float fa = fmaf(c.half_dt, u, a), fb = fmaf(c.half_dt, v, bb);
             float n1 = 0.0f, n2 = 0.0f;
             if (NM == NOISE_ARRAY) {
-                n1 = buf_load<float>(at(reinterpret_cast<const char*>(noise1), x), off[r]);
-                if (noise2) n2 = buf_load<float>(at(reinterpret_cast<const char*>(noise2), x), off[r]);
+                n1 = buf_load_s(at(reinterpret_cast<const char*>(noise1), x), vo, so[r]);
+                if (noise2) n2 = buf_load_s(at(reinterpret_cast<const char*>(noise2), x), vo, so[r]);
             }
             float g1 = 0.0f, g2 = 0.0f;
             if (NM == NOISE_HASH)
                 gauss_pair_f32_l2((col0K + (uint64_t)r * rowK + xK) ^ noise_seed, c.word_mask, g1,
                                   g2);
             float d1, d2;
-            complex_force_f32(rad, zn, a, bb, d1, d2);
+            if constexpr (Radial::kEarly)
+                complex_force_finish(rad, zn, a, bb, rho1[r], P1[r], d1, d2);
+            else
+                complex_force_f32(rad, zn, a, bb, d1, d2);
             const float k1a = fmaf(-c.inv_mu2, d1, fmaf(c.c_lap, t1a, -c.eta * u));
             const float k1b = fmaf(-c.inv_mu2, d2, fmaf(c.c_lap, t1b, -c.eta * v));
             const float ga = fmaf(c.half_dt, k1a, u), gb = fmaf(c.half_dt, k1b, v);
             const float t2a = fmaf(c.half_dt, fmaf(-6.0f, u, sum[2]), t1a);
             const float t2b = fmaf(c.half_dt, fmaf(-6.0f, v, sum[3]), t1b);
             float e1, e2;
-            complex_force_f32(rad, zn, fa, fb, e1, e2);
+            if constexpr (Radial::kEarly)
+                complex_force_finish(rad, zn, fa, fb, rho2[r], P2[r], e1, e2);
+            else
+                complex_force_f32(rad, zn, fa, fb, e1, e2);
             const float k2a = fmaf(-c.inv_mu2, e1, fmaf(c.c_lap, t2a, -c.eta * ga));
             const float k2b = fmaf(-c.inv_mu2, e2, fmaf(c.c_lap, t2b, -c.eta * gb));
             const float na = fmaf(c.dt, ga, a), nb = fmaf(c.dt, gb, bb);
@@ -957,10 +1015,10 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 pa = fmaf(c.dt, k2a, u);
                 pbv = fmaf(c.dt, k2b, v);
             }
-            buf_store(na, at(O[0], x), offs[r]);
-            buf_store(nb, at(O[1], x), offs[r]);
-            buf_store(pa, at(O[2], x), offs[r]);
-            buf_store(pbv, at(O[3], x), offs[r]);
+            buf_store_s(na, at(O[0], x), vs, so[r]);
+            buf_store_s(nb, at(O[1], x), vs, so[r]);
+            buf_store_s(pa, at(O[2], x), vs, so[r]);
+            buf_store_s(pbv, at(O[3], x), vs, so[r]);
             if (count) cnt += wave_count(act && fmaf(na, na, nb * nb) > esc2);
         }
         xK += planeK;

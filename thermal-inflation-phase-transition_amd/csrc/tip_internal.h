@@ -164,6 +164,22 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uin
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
+// The same with a wave-uniform byte offset in a scalar register (soffset), so row offsets that
+// are equal across the wave cost no vector registers.  Callers keep voffset + soffset inside
+// the resource's range for every lane that must not be dropped, and out-of-range lanes at or
+// past the range without 32-bit wrap-around.
+__device__ __forceinline__ float buf_load_s(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_store_s(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                            uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff,
+                                          (int)soff, 0);
+}
+
 __device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);

@@ -246,7 +246,8 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
         if (rs.ok) {
             TileGrid tg;
             tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
-            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
+            // about 2048 blocks (measured best at 256^3: 16 planes per block with TY = 2)
+            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 2048;
             if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
             TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
             tg.cx = (int)cx;
