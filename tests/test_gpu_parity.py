@@ -354,6 +354,26 @@ def test_winding_bit_exact(shape):
     assert np.array_equal(plaq2, plaq_ref) and nstr2 == nstr
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 3, 5), (2, 2, 2), (3, 1, 65), (17, 5, 3),
+                                   (33, 4, 130)])
+def test_winding_small_and_odd_extents(shape):
+    """x extents of 1, 2, 3 planes (the loads of plane x+2 wrap around the lattice while a
+    block marches its chunk), chunks that end mid-lattice (17, 33 planes: 16-plane chunks plus
+    a remainder), y extents of 1 (the y neighbour is the site itself) and z rows that are not
+    a multiple of the 64-lane wave."""
+    tip, K, KC, DA, _ = _tip()
+    rng = np.random.default_rng(sum(shape) + 100)
+    a1 = rng.normal(size=shape).astype(np.float32)
+    a2 = rng.normal(size=shape).astype(np.float32)
+    w_ref, plaq_ref = O.winding(a1, a2)
+    for dtype in (np.float32, np.float64):  # float64 inputs hold the same (float32) values
+        p1, p2 = a1.astype(dtype), a2.astype(dtype)
+        w, nstr, nplq, plaq = KC.compute_winding_number(p1, p2, plaquettes=True)
+        assert np.array_equal(plaq, plaq_ref), (shape, dtype)
+        assert np.array_equal(w, np.rint(w_ref).astype(np.float32))
+        assert nstr == O.string_count(w_ref) and nplq == int(np.count_nonzero(plaq_ref))
+
+
 def test_winding_degenerate_edges():
     """Edges whose cross product is exactly zero: zero vectors with signed zeros (angle +-pi),
     the two sides of the negative real axis (theta = pi and -pi) and antiparallel pairs; the
@@ -396,7 +416,7 @@ def test_vacuum_counts_exact():
     assert K.vacuum_count(a, 1e4, b) == O.vacuum_count(a, 1e4, b)
 
 
-@pytest.mark.parametrize("n", [1, 3, 4097, 1 << 22, (1 << 22) + 3, 9_000_001])
+@pytest.mark.parametrize("n", [0, 1, 3, 4097, 1 << 22, (1 << 22) + 3, 9_000_001])
 def test_vacuum_counts_sizes_and_alignment(n):
     """Counts of 16-byte vector loads plus the element tail, several chunks per thread at the
     largest sizes, and fields that start off a 16-byte boundary (element loads)."""
