@@ -201,10 +201,10 @@ def bench_ensemble(steps: int = 200):
 
 
 def bench_extras(steps: int = 200) -> dict:
-    """The other forms on one GPU: complex 256^3 float32 step and winding,
-    the float64 drop-in real 256^3 step, and the complex 512^3 step with the inline MLP
-    surrogate (the 7350 -> 6770 K cooling workload's lattice; throughput does not depend on the weights, untrained ones are
-    used)."""
+    """The other forms on one GPU: complex 256^3 float32 step and winding, the float64 drop-in
+    real 256^3 step, the complex 512^3 step with the inline MLP surrogate (the 7350 -> 6770 K
+    cooling workload's lattice; throughput does not depend on the weights, untrained ones are
+    used), the BAOAB and overdamped steps and the component labelling."""
     from tip import potential as P
     from tip import surrogate as S
     from tip.lattice import Lattice

@@ -980,7 +980,8 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             float n1 = 0.0f, n2 = 0.0f;
             if (NM == NOISE_ARRAY) {
                 n1 = buf_load_s(at(reinterpret_cast<const char*>(noise1), x), vo, so[r]);
-                if (noise2) n2 = buf_load_s(at(reinterpret_cast<const char*>(noise2), x), vo, so[r]);
+                if (noise2)
+                    n2 = buf_load_s(at(reinterpret_cast<const char*>(noise2), x), vo, so[r]);
             }
             float g1 = 0.0f, g2 = 0.0f;
             if (NM == NOISE_HASH)
