@@ -489,6 +489,47 @@ def test_lattice_geometries_bitwise_identical():
     assert np.array_equal(g, ref[0]) and np.array_equal(h, ref[1])
 
 
+@pytest.mark.parametrize("complex_field,shape", [(False, (3, 4, 1000)), (False, (3, 4, 450)),
+                                                 (True, (3, 4, 512)), (True, (3, 4, 450)),
+                                                 (True, (5, 6, 66))])
+def test_row_tiles_wide_and_ragged_z(complex_field, shape):
+    """Row tiles whose z rows span many waves (up to 16 real / 8 complex) and end in a wave
+    with only a few active lanes (450 = 7 x 64 + 2, 66 = 64 + 2), whose idle lanes store past
+    the plane's end: every row count gives identical fields, and the flat tile agrees to float32
+    rounding."""
+    from tip import potential as P
+    from tip.lattice import Lattice, KERNEL_FLAT, KERNEL_ROWS
+    rng = np.random.default_rng(sum(shape) + complex_field)
+    names = ("phi1", "phi2", "pi1", "pi2") if complex_field else ("phi", "pi")
+    init = {k: (rng.normal(size=shape) * (2e3 if k.startswith("phi") else 1e2)).astype(np.float32)
+            for k in names}
+    if complex_field:
+        t, tmin, dinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, P.ModelParams.complex())
+        kw = dict(dt=0.25, dx=1.0, eta_eff=1.0, mu=1000.0, noise_scale=300.0)
+    else:
+        t, tmin, dinv = real_table(1024)
+        prm = real_params()
+        kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+                  noise_scale=prm["noise_scale"])
+    outs = {}
+    geos = [(1, 2, 1, KERNEL_ROWS), (2, 0, 0, KERNEL_ROWS), (1, 0, 1, KERNEL_FLAT)]
+    for geo in geos:
+        lat = Lattice(shape, np.float32, complex_field=complex_field)
+        try:
+            for k, v in init.items():
+                lat.upload(k, v)
+            lat.set_table(t, tmin, dinv, odd=not complex_field)
+            lat.set_geometry(*geo)
+            lat.step(4, seed=6, step0=0, **kw)
+            outs[geo] = [lat.download(k) for k in names]
+        finally:
+            lat.close()
+    a, b, f = (outs[g] for g in geos)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    for x, y in zip(a, f):
+        assert rel(x.astype(np.float64), y.astype(np.float64)) < 1e-5
+
+
 def test_full_size_256_uniform_mode_closed_form():
     """256^3 float32, no noise, linear table: a uniform field follows the exact RK2 matrix."""
     from tip.lattice import Lattice
