@@ -342,12 +342,12 @@ __device__ __forceinline__ unsigned rk2_real_tile_f32(
 // ny; groups past the last row are idle.  The next plane's centres and halo rows are
 // loaded one plane ahead.
 // ---------------------------------------------------------------------------------------
-template <bool ODD, int TY, int NM>
+template <bool ODD, int TY, int NM, bool CNT>
 __device__ __forceinline__ unsigned rk2_real_rows_tile(
     const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
     float* __restrict__ pi_out, const float* __restrict__ noise, const StepScalars<float>& c,
     const TableF32<ODD>& tab, int nx, int ny, int nz, int W, int tile, int x0, int x1,
-    uint64_t noise_seed, bool count, float esc, float* __restrict__ lds)
+    uint64_t noise_seed, float esc, float* __restrict__ lds)
 {
     // LDS words per z column: 2 buffers x 2 fields x TY rows, plus one pad word so that
     // consecutive lanes (stride K words) hit distinct banks.  Every slot of a column is a
@@ -464,7 +464,7 @@ __device__ __forceinline__ unsigned rk2_real_rows_tile(
             else qnew = fmaf(c.dt, k2, qc[r]);
             buf_store(pnew, at(PO, x), offs[r]);
             buf_store(qnew, at(QO, x), offs[r]);
-            if (count) cnt += wave_count(act && fabsf(pnew) > esc);
+            if constexpr (CNT) cnt += wave_count(act && fabsf(pnew) > esc);
         }
         xK += planeK;
     };
@@ -721,7 +721,9 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_complex(
     if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
-template <bool ODD, int TY, int NM>
+// CNT: also count the sites with |phi_new| > esc into *vac (a separate instantiation, so the
+// plain step carries no per-row test of the flag).
+template <bool ODD, int TY, int NM, bool CNT>
 __global__ __launch_bounds__(1024) void k_rk2_real_rows(
     const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
     float* __restrict__ pi_out, const float* __restrict__ noise, StepScalars<float> c,
@@ -733,10 +735,10 @@ __global__ __launch_bounds__(1024) void k_rk2_real_rows(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_real_rows_tile<ODD, TY, NM>(phi, pi, phi_out, pi_out, noise, c, tab, nx,
-                                                   ny, nz, W, tp, x0, x1, noise_seed,
-                                                   vac != nullptr, esc, lds_rows);
-    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+    unsigned cnt = rk2_real_rows_tile<ODD, TY, NM, CNT>(phi, pi, phi_out, pi_out, noise, c, tab,
+                                                        nx, ny, nz, W, tp, x0, x1, noise_seed,
+                                                        esc, lds_rows);
+    if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_ensemble(
 
 // Row-tile variant (LDS staging, y rows per thread TY): a block is 4 waves; an instance with
 // W waves per z row gets 4/W row groups per block.
-template <int TY, bool ODD>
+template <int TY, bool ODD, bool CNT>
 __global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
     const EnsInst* __restrict__ insts, const EnsItem* __restrict__ items, int nitems, int cx,
     int parity, const uint64_t* __restrict__ step_base, int step_off, int xcd_remap,
@@ -96,11 +96,10 @@ __global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
         tab.off = I.off;
         const int x1 = min(it.x0 + cx, I.nx);
         __syncthreads();  // the previous item's last plane may still be read from LDS
-        unsigned cnt = rk2_real_rows_tile<ODD, TY, NOISE_HASH>(
+        unsigned cnt = rk2_real_rows_tile<ODD, TY, NOISE_HASH, CNT>(
             I.f[parity][0], I.f[parity][1], I.f[parity ^ 1][0], I.f[parity ^ 1][1], nullptr, I.c,
-            tab, I.nx, I.ny, I.nz, I.W, it.tp, it.x0, x1, step ^ I.seed_mix, vac != nullptr, esc,
-            lds_ens);
-        if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
+            tab, I.nx, I.ny, I.nz, I.W, it.tp, it.x0, x1, step ^ I.seed_mix, esc, lds_ens);
+        if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
     }
 }
 
@@ -286,10 +285,16 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
     const int remap = (E->xcd && nb % 8 == 0) ? 1 : 0;
     if (E->use_rows) {
         const size_t lds = (size_t)256 * (4 * E->ty + 1) * sizeof(float);
-#define TIP_LAUNCH_ENSR(TYV, ODD)                                                                \
-    hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD>), dim3(nb), dim3(64, 4), lds, E->stream, \
-                       E->dev_inst, items, nit, E->cx, parity, E->dev_step,                     \
+#define TIP_LAUNCH_ENSR_C(TYV, ODD, CNT)                                                      \
+    hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD, CNT>), dim3(nb), dim3(64, 4), lds,       \
+                       E->stream, E->dev_inst, items, nit, E->cx, parity, E->dev_step,          \
                        step_off, remap, vac, esc)
+#define TIP_LAUNCH_ENSR(TYV, ODD)                \
+    if (vac) {                                   \
+        TIP_LAUNCH_ENSR_C(TYV, ODD, true);       \
+    } else {                                     \
+        TIP_LAUNCH_ENSR_C(TYV, ODD, false);      \
+    }
 #define TIP_LAUNCH_ENSR_TY(ODD)                        \
     switch (E->ty) {                                   \
         case 4: TIP_LAUNCH_ENSR(4, ODD); break;        \
@@ -303,6 +308,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
         }
 #undef TIP_LAUNCH_ENSR_TY
 #undef TIP_LAUNCH_ENSR
+#undef TIP_LAUNCH_ENSR_C
         TIP_HIP(hipGetLastError());
         return TIP_OK;
     }

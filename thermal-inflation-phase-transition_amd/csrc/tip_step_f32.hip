@@ -157,10 +157,16 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
             tg.xcd_remap = ((geom ? geom->xcd : 1) && nbr % 8 == 0) ? 1 : 0;
             const size_t lds = (size_t)rs.G * (64 * rs.W) * (4 * rs.TY + 1) * sizeof(float);
             const dim3 blk(64, rs.W * rs.G);
-#define TIP_LAUNCH_ROWS(TYV, ODD, NM)                                                           \
-    hipLaunchKernelGGL((k_rk2_real_rows<ODD, TYV, NM>), dim3(nbr), blk, lds, st, phi, pi,     \
+#define TIP_LAUNCH_ROWS_C(TYV, ODD, NM, CNT)                                                    \
+    hipLaunchKernelGGL((k_rk2_real_rows<ODD, TYV, NM, CNT>), dim3(nbr), blk, lds, st, phi, pi, \
                        phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny, (int)nz,    \
                        rs.W, tg, noise_seed, vac, esc)
+#define TIP_LAUNCH_ROWS(TYV, ODD, NM)                 \
+    if (vac) {                                        \
+        TIP_LAUNCH_ROWS_C(TYV, ODD, NM, true);        \
+    } else {                                          \
+        TIP_LAUNCH_ROWS_C(TYV, ODD, NM, false);       \
+    }
 #define TIP_LAUNCH_ROWS_NM(TYV, ODD)                                      \
     switch (nm) {                                                         \
         case NOISE_ARRAY: TIP_LAUNCH_ROWS(TYV, ODD, NOISE_ARRAY); break;  \
@@ -181,6 +187,7 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
 #undef TIP_LAUNCH_ROWS_TY
 #undef TIP_LAUNCH_ROWS_NM
 #undef TIP_LAUNCH_ROWS
+#undef TIP_LAUNCH_ROWS_C
             TIP_HIP(hipGetLastError());
             return TIP_OK;
         }
