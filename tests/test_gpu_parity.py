@@ -250,6 +250,37 @@ def test_fused_vacuum_count_matches_standalone():
     assert lo == float(host.min()) and hi == float(host.max())
 
 
+def test_fused_vacuum_count_float64_and_complex():
+    """The counting instantiations of the float64 step and of the complex row-tile step: the
+    fused count equals the standalone count of the new field (complex: up to the sites whose
+    rho^2 lies within float32 rounding of esc^2, where the fused float32 test and the float64
+    standalone test may differ)."""
+    tip, K, KC, DA, Counter = _tip()
+    rng = np.random.default_rng(12)
+    shape = (16, 24, 64)
+    t, tmin, dinv = -1e6 * np.linspace(-1e6, 1e6, 513), -1e6, 512 / 2e6
+    phi = DA.from_numpy(rng.normal(size=shape) * 2e4)
+    pi = DA.from_numpy(rng.normal(size=shape) * 1e3)
+    pt, qt = DA(shape, np.float64), DA(shape, np.float64)
+    c = Counter(1)
+    K.rk2_step_table(phi, pi, 0.25, 2.0, 7.0, 1000.0, None, pt, qt, 1.0, t, tmin, dinv, len(t),
+                     noise_seed=3, noise_scale=100.0, vac_count=c, vac_esc=1e4)
+    assert int(c.read()[0]) == K.vacuum_count(phi, 1e4)
+
+    from tip import potential as P
+    ct, ctmin, cdinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, P.ModelParams.complex())
+    f = [DA.from_numpy((rng.normal(size=shape) * 1e4).astype(np.float32)) for _ in range(4)]
+    tmp = [DA(shape, np.float32) for _ in range(4)]
+    c = Counter(1)
+    KC.rk2_step_table(*f, 0.25, 1.0, 1.0, 1000.0, None, None, *tmp, 1.0, ct, ctmin, cdinv,
+                      len(ct), noise_seed=5, noise_scale=300.0, vac_count=c, vac_esc=1e4)
+    fused = int(c.read()[0])
+    a, b = f[0].download().astype(np.float64), f[1].download().astype(np.float64)
+    r2 = a * a + b * b
+    near = int(np.sum(np.abs(r2 - 1e8) < 1e8 * 2.5e-7))
+    assert abs(fused - K.vacuum_count(f[0], 1e4, f[1])) <= near
+
+
 # ----------------------------------------------------------------------------------------
 # complex field
 # ----------------------------------------------------------------------------------------

@@ -78,13 +78,13 @@ __device__ __forceinline__ unsigned wave_count(bool p)
 // [x0, x1).  Flattening keeps every lane busy whatever nz is (a 96-wide row would leave a
 // quarter of a 64 x k tile idle).  Lanes past the plane compute on site 0 and store nothing,
 // so loads need no predication.  Returns the wave's count of sites with |phi_new| > esc when
-// `count` is set (all lanes of every wave call it with the same x0, x1 and count).
-template <typename R, typename Tab, int BY>
+// CNT is set (all lanes of every wave call it with the same x0 and x1).
+template <typename R, typename Tab, int BY, bool CNT>
 __device__ __forceinline__ unsigned rk2_real_tile(
     const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
     R* __restrict__ pi_out, const R* __restrict__ noise, const StepScalars<R>& c,
     const Tab& tab, int nx, int ny, int nz, int tile_p, int x0, int x1,
-    uint64_t noise_seed, int hash_noise, bool count, R esc)
+    uint64_t noise_seed, int hash_noise, R esc)
 {
     const int nyz = ny * nz;
     const int pl = tile_p * (64 * BY) + (int)threadIdx.y * 64 + (int)threadIdx.x;
@@ -167,7 +167,7 @@ __device__ __forceinline__ unsigned rk2_real_tile(
             buf_store(pn, at(PO, x), o_c);
             buf_store(qn, at(QO, x), o_c);
         }
-        if (count) cnt += wave_count(act && r_abs(pn) > esc);
+        if constexpr (CNT) cnt += wave_count(act && r_abs(pn) > esc);
         pm = pc; qm = qc; pc = pp; qc = qp;
         xK += planeK;
     }
@@ -516,7 +516,7 @@ struct TileGrid {
     }
 };
 
-template <typename R, typename Tab, int BY>
+template <typename R, typename Tab, int BY, bool CNT>
 __global__ __launch_bounds__(64 * BY) void k_rk2_real(
     const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
     R* __restrict__ pi_out, const R* __restrict__ noise, StepScalars<R> c, Tab tab, int nx,
@@ -527,10 +527,10 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_real(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_real_tile<R, Tab, BY>(phi, pi, phi_out, pi_out, noise, c, tab, nx, ny, nz,
-                                             tp, x0, x1, noise_seed, hash_noise, vac != nullptr,
-                                             esc);
-    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+    unsigned cnt = rk2_real_tile<R, Tab, BY, CNT>(phi, pi, phi_out, pi_out, noise, c, tab, nx,
+                                                  ny, nz, tp, x0, x1, noise_seed, hash_noise,
+                                                  esc);
+    if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
 template <bool ODD, int BY, int NM>
@@ -862,14 +862,14 @@ __device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnPar
     complex_force_finish(rad, zn, a, b, rho, rad.fetch(rho), d1, d2);
 }
 
-template <int TY, int NM, typename Radial>
+template <int TY, int NM, typename Radial, bool CNT>
 __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
     float* __restrict__ o3, float* __restrict__ o4, const float* __restrict__ noise1,
     const float* __restrict__ noise2, const StepScalars<float>& c, const Radial& rad,
     const ZnParams<float>& zn, int nx, int ny, int nz, int W, int tile, int x0, int x1,
-    uint64_t noise_seed, bool count, float esc2, float* __restrict__ lds)
+    uint64_t noise_seed, float esc2, float* __restrict__ lds)
 {
     constexpr int NF = 4;                 // phi1, phi2, pi1, pi2
     constexpr int K = 2 * NF * TY + 1;    // LDS words per z column
@@ -1022,7 +1022,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             buf_store_s(nb, at(O[1], x), vs, so[r]);
             buf_store_s(pa, at(O[2], x), vs, so[r]);
             buf_store_s(pbv, at(O[3], x), vs, so[r]);
-            if (count) cnt += wave_count(act && fmaf(na, na, nb * nb) > esc2);
+            if constexpr (CNT) cnt += wave_count(act && fmaf(na, na, nb * nb) > esc2);
         }
         xK += planeK;
     };
@@ -1051,7 +1051,8 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     return cnt;
 }
 
-template <int TY, int NM, typename Radial>
+// CNT: also count the sites with |phi_new|^2 > esc2 into *vac (separate instantiation).
+template <int TY, int NM, typename Radial, bool CNT>
 __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
@@ -1065,10 +1066,10 @@ __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial>(
+    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT>(
         phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, rad, zn, nx, ny, nz, W, tp, x0,
-        x1, noise_seed, vac != nullptr, esc2, lds_crows);
-    if (vac && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
+        x1, noise_seed, esc2, lds_crows);
+    if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }
 
 // Launch geometry shared by the real and complex launchers.

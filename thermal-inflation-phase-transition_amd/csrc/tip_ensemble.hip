@@ -290,11 +290,12 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
                        E->stream, E->dev_inst, items, nit, E->cx, parity, E->dev_step,          \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENSR(TYV, ODD)                \
-    if (vac) {                                   \
-        TIP_LAUNCH_ENSR_C(TYV, ODD, true);       \
-    } else {                                     \
-        TIP_LAUNCH_ENSR_C(TYV, ODD, false);      \
-    }
+    do {                                         \
+        if (vac)                                 \
+            TIP_LAUNCH_ENSR_C(TYV, ODD, true);   \
+        else                                     \
+            TIP_LAUNCH_ENSR_C(TYV, ODD, false);  \
+    } while (0)
 #define TIP_LAUNCH_ENSR_TY(ODD)                        \
     switch (E->ty) {                                   \
         case 4: TIP_LAUNCH_ENSR(4, ODD); break;        \

@@ -397,10 +397,17 @@ int launch_complex_mlp_grid(const float* phi1, const float* phi2, const float* p
     tg.xcd_remap = nb64 % 8 == 0 ? 1 : 0;
     const size_t lds = (size_t)rs.G * (64 * rs.W) * (8 * rs.TY + 1) * sizeof(float);
     const dim3 blk(64, rs.W * rs.G);
-#define TIP_LAUNCH_MG(TYV, NM)                                                                   \
-    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, MlpGridRadial>), dim3((int)nb64), blk, lds, \
-                       st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, rad, zn,  \
-                       (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_MG_C(TYV, NM, CNT)                                                          \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, MlpGridRadial, CNT>), dim3((int)nb64), blk, \
+                       lds, st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, rad, \
+                       zn, (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_MG(TYV, NM)                \
+    do {                                      \
+        if (vac)                              \
+            TIP_LAUNCH_MG_C(TYV, NM, true);   \
+        else                                  \
+            TIP_LAUNCH_MG_C(TYV, NM, false);  \
+    } while (0)
     if (rs.TY == 2) {
         if (hash) TIP_LAUNCH_MG(2, NOISE_HASH);
         else TIP_LAUNCH_MG(2, NOISE_NONE);

@@ -162,11 +162,12 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
                        phi_out, pi_out, noise, c, tab.as<ODD>(), (int)nx, (int)ny, (int)nz,    \
                        rs.W, tg, noise_seed, vac, esc)
 #define TIP_LAUNCH_ROWS(TYV, ODD, NM)                 \
-    if (vac) {                                        \
-        TIP_LAUNCH_ROWS_C(TYV, ODD, NM, true);        \
-    } else {                                          \
-        TIP_LAUNCH_ROWS_C(TYV, ODD, NM, false);       \
-    }
+    do {                                              \
+        if (vac)                                      \
+            TIP_LAUNCH_ROWS_C(TYV, ODD, NM, true);    \
+        else                                          \
+            TIP_LAUNCH_ROWS_C(TYV, ODD, NM, false);   \
+    } while (0)
 #define TIP_LAUNCH_ROWS_NM(TYV, ODD)                                      \
     switch (nm) {                                                         \
         case NOISE_ARRAY: TIP_LAUNCH_ROWS(TYV, ODD, NOISE_ARRAY); break;  \
@@ -267,10 +268,17 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             const dim3 blk(64, rs.W * rs.G);
             TableRadial t32;
             t32.t = tab.as<false>();
-#define TIP_LAUNCH_CROWS(TYV, NM)                                                                \
-    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial>), dim3(nbr), blk, lds, st, phi1, \
-                       phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn, (int)nx,      \
-                       (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_CROWS_C(TYV, NM, CNT)                                                      \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT>), dim3(nbr), blk, lds, st, \
+                       phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn,        \
+                       (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_CROWS(TYV, NM)              \
+    do {                                       \
+        if (vac)                               \
+            TIP_LAUNCH_CROWS_C(TYV, NM, true); \
+        else                                   \
+            TIP_LAUNCH_CROWS_C(TYV, NM, false);\
+    } while (0)
 #define TIP_LAUNCH_CROWS_NM(TYV)                                      \
     switch (nm) {                                                     \
         case NOISE_ARRAY: TIP_LAUNCH_CROWS(TYV, NOISE_ARRAY); break;  \
@@ -284,6 +292,7 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             }
 #undef TIP_LAUNCH_CROWS_NM
 #undef TIP_LAUNCH_CROWS
+#undef TIP_LAUNCH_CROWS_C
             TIP_HIP(hipGetLastError());
             return TIP_OK;
         }

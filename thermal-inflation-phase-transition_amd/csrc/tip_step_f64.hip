@@ -39,10 +39,15 @@ int launch_real_f64(const double* phi, const double* pi, double* phi_out, double
     rc = resolve_geometry(nullptr, nx, ny, nz, 0, 1, g, tg, nb);
     if (rc) return rc;
     int hash = (noise == nullptr && noise_scale != 0.0) ? 1 : 0;
-    hipLaunchKernelGGL((k_rk2_real<double, TableF64, 4>), dim3(nb), dim3(64, 4), 0, st, phi, pi,
-                       phi_out, pi_out, noise, c,
-                       make_table(vp_table, vp_tmin, vp_dinv, vp_npts, vp_odd), (int)nx, (int)ny,
-                       (int)nz, tg, noise_seed, hash, vac, vac_esc);
+    const TableF64 tab = make_table(vp_table, vp_tmin, vp_dinv, vp_npts, vp_odd);
+    if (vac)
+        hipLaunchKernelGGL((k_rk2_real<double, TableF64, 4, true>), dim3(nb), dim3(64, 4), 0, st,
+                           phi, pi, phi_out, pi_out, noise, c, tab, (int)nx, (int)ny, (int)nz, tg,
+                           noise_seed, hash, vac, vac_esc);
+    else
+        hipLaunchKernelGGL((k_rk2_real<double, TableF64, 4, false>), dim3(nb), dim3(64, 4), 0, st,
+                           phi, pi, phi_out, pi_out, noise, c, tab, (int)nx, (int)ny, (int)nz, tg,
+                           noise_seed, hash, vac, vac_esc);
     TIP_HIP(hipGetLastError());
     return TIP_OK;
 }
