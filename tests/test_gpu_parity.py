@@ -447,6 +447,48 @@ def test_vacuum_counts_exact():
     assert K.vacuum_count(a, 1e4, b) == O.vacuum_count(a, 1e4, b)
 
 
+@pytest.mark.parametrize("n", [1, 3, 4097, (1 << 22) + 3, 9_000_001])
+def test_minmax_sizes_alignment_and_nan(n):
+    """Field min/max (float32, float64) and the complex radius max: exact at several chunk
+    counts, with aligned (16-byte loads) and misaligned (element loads) fields, and a NaN
+    anywhere reaches both ends."""
+    tip, K, KC, DA, Counter = _tip()
+    from tip import _lib
+    rng = np.random.default_rng(n + 7)
+    out = DA((2,), np.float64)
+    try:
+        for dtype, name in ((np.float32, "tip_minmax_f32"), (np.float64, "tip_minmax_f64")):
+            phi = (rng.normal(size=n + 1) * 1e4).astype(dtype)
+            d = DA.from_numpy(phi)
+            try:
+                for off in (0, 1):
+                    _lib.call(name, d.ptr + off * phi.itemsize, n, out.ptr, None)
+                    r = out.download()
+                    v = phi[off:off + n]
+                    assert r[0] == float(v.min()) and r[1] == float(v.max()), (dtype, off)
+                phi[n // 2] = np.nan
+                d.upload(phi)
+                _lib.call(name, d.ptr, n, out.ptr, None)
+                r = out.download()
+                assert np.isnan(r[0]) and np.isnan(r[1])
+            finally:
+                d.free()
+        a = (rng.normal(size=n + 1) * 1e4).astype(np.float32)
+        b = (rng.normal(size=n + 1) * 1e4).astype(np.float32)
+        da, db = DA.from_numpy(a), DA.from_numpy(b)
+        try:
+            for oa, ob in ((0, 0), (1, 1), (0, 1)):
+                rho = np.sqrt(a[oa:oa + n].astype(np.float64) ** 2
+                              + b[ob:ob + n].astype(np.float64) ** 2)
+                _lib.call("tip_rho_max_c32", da.ptr + 4 * oa, db.ptr + 4 * ob, n, out.ptr, None)
+                assert out.download()[0] == float(rho.max())
+        finally:
+            da.free()
+            db.free()
+    finally:
+        out.free()
+
+
 @pytest.mark.parametrize("n", [0, 1, 3, 4097, 1 << 22, (1 << 22) + 3, 9_000_001])
 def test_vacuum_counts_sizes_and_alignment(n):
     """Counts of 16-byte vector loads plus the element tail, several chunks per thread at the

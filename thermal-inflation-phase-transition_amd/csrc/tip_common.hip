@@ -633,25 +633,51 @@ extern "C" int tip_vacuum_count_c32(const float* phi1, const float* phi2, int64_
 
 // Two-stage min/max: per-block partials into scratch, then one block folds them.
 // mode 0: min/max of a; mode 1: max of sqrt(a^2 + b^2) (written to out[0]); mode 2: min/max of
-// sqrt(a^2 + b^2).
-template <typename R, int MODE>
+// sqrt(a^2 + b^2).  The partial pass reads like the vacuum count (kVacU independent 16-byte
+// loads per field per thread before any is used; element loads for unaligned fields).
+template <typename R, int MODE, int L>
 __global__ __launch_bounds__(256) void k_minmax_part(const R* __restrict__ a,
                                                      const R* __restrict__ b, int64_t n,
                                                      double* part)
 {
     __shared__ double smin[256], smax[256];
     double lo = INFINITY, hi = -INFINITY;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        double v;
-        if (MODE == 0) v = (double)a[s];
-        else {  // modes 1, 2: rho
-            double u = (double)a[s], w = (double)b[s];
-            v = sqrt(u * u + w * w);
-        }
-        // NaN propagates to both ends so the caller's finiteness check sees it
+    // NaN propagates to both ends so the caller's finiteness check sees it
+    auto take = [&](double v) {
         lo = (v < lo || v != v) ? v : lo;
         hi = (v > hi || v != v) ? v : hi;
+    };
+    auto value = [&](R u, R w) -> double {
+        if (MODE == 0) return (double)u;
+        const double x = (double)u, y = (double)w;  // modes 1, 2: rho
+        return sqrt(x * x + y * y);
+    };
+    const int64_t nv = n / L;
+    const int64_t span = (int64_t)gridDim.x * blockDim.x * kVacU;
+    for (int64_t s0 = (int64_t)blockIdx.x * blockDim.x * kVacU + threadIdx.x; s0 < nv;
+         s0 += span) {
+        Pack<R, L> x[kVacU], y[kVacU];
+#pragma unroll
+        for (int u = 0; u < kVacU; ++u) {
+            const int64_t s = s0 + (int64_t)u * blockDim.x;
+            x[u] = Pack<R, L>{};
+            y[u] = Pack<R, L>{};
+            if (s < nv) {
+                x[u] = load_pack<R, L>(a, s);
+                if (MODE != 0) y[u] = load_pack<R, L>(b, s);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kVacU; ++u) {
+            if (s0 + (int64_t)u * blockDim.x < nv) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) take(value(x[u].e[l], y[u].e[l]));
+            }
+        }
+    }
+    if (blockIdx.x == 0 && (int64_t)threadIdx.x < n - nv * L) {
+        const int64_t s = nv * L + threadIdx.x;
+        take(value(a[s], MODE != 0 ? b[s] : (R)0));
     }
     smin[threadIdx.x] = lo;
     smax[threadIdx.x] = hi;
@@ -707,12 +733,21 @@ template <typename R, int MODE>
 static int minmax(const R* a, const R* b, int64_t n, double* out, void* stream)
 {
     TIP_REQUIRE(a && out && n > 0, "tip_minmax: bad arguments");
-    unsigned nb = reduce_blocks(n);
-    if (nb > 1024) nb = 1024;
+    constexpr int L = 16 / sizeof(R);
+    const bool vec = ((uintptr_t)a % 16 == 0) && (MODE == 0 || (uintptr_t)b % 16 == 0);
+    const int64_t nv = vec ? n / L : n;
+    int64_t nb64 = (nv + 256 * kVacU - 1) / (256 * kVacU);
+    nb64 = nb64 < 1 ? 1 : (nb64 > 1024 ? 1024 : nb64);
+    const unsigned nb = (unsigned)nb64;
     double* part = reinterpret_cast<double*>(scratch_get(sizeof(double) * 2 * nb, 1));
     TIP_REQUIRE(part, "tip_minmax: scratch allocation failed");
     hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL((k_minmax_part<R, MODE>), dim3(nb), dim3(256), 0, st, a, b, n, part);
+    if (vec)
+        hipLaunchKernelGGL((k_minmax_part<R, MODE, L>), dim3(nb), dim3(256), 0, st, a, b, n,
+                           part);
+    else
+        hipLaunchKernelGGL((k_minmax_part<R, MODE, 1>), dim3(nb), dim3(256), 0, st, a, b, n,
+                           part);
     TIP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_minmax_fold, dim3(1), dim3(256), 0, st, part, (int)nb, out, MODE);
     TIP_HIP(hipGetLastError());
