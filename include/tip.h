@@ -414,11 +414,14 @@ int tip_ensemble_set_params(tip_ensemble *ens, int inst, const tip_step_params *
  * each walking a contiguous item range (0 = one block per item). */
 int tip_ensemble_set_geometry(tip_ensemble *ens, int kernel, int rows, int planes_per_block,
                               int xcd_remap, int persistent_blocks);
-/* Step schedule of the batched modes: with group_bytes > 0 the instances are cut, in order,
- * into groups holding at most group_bytes of field data (phi and pi, both buffers: 16 B per
- * site) and each group runs all nsteps of a call before the next group starts, so a group's
- * fields stay in the GPU's last-level cache (Infinity Cache) from one step to the next.
- * 0 (default) = one launch per step over every instance.  Results are identical either way. */
+/* Step schedule of the batched modes.  The row kernel launches instances whose z extent
+ * divides 256, those whose z extent divides 192 (rows packed densely in 3-wave blocks) and the
+ * rest separately.  With group_bytes > 0 each such class of B bytes of field data (phi and
+ * pi, both buffers: 16 B per site) is cut into ceil(B / group_bytes) groups of about equal
+ * bytes (contiguous runs of its instances, a group holding one instance at least), and each
+ * group runs all nsteps of a call before the next group starts, so a group's fields stay in
+ * the GPU's last-level cache (Infinity Cache) from one step to the next.  0 (default) = one
+ * launch per step per class.  Results are identical either way. */
 int tip_ensemble_set_schedule(tip_ensemble *ens, int64_t group_bytes);
 int tip_ensemble_step(tip_ensemble *ens, int64_t nsteps, int mode, float *elapsed_ms);
 int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long *host_counts);

@@ -12,19 +12,23 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 64), (24, 20, 70), (8, 8, 8), (12, 33, 65)]
+# z extents of every row mapping of the ensemble's row kernel: 64 and 8 (dense rows in
+# 256-thread blocks), 96 and 24 (dense rows in 192-thread blocks), 70 and 65 (whole waves per
+# row); the classes are interleaved so that the stepping order differs from the instance order
+SHAPES = [(16, 16, 64), (24, 20, 70), (10, 12, 96), (8, 8, 8), (12, 33, 65), (6, 8, 24)]
+SEEDS = (3, 11, 5, 0, 7, 9)
 
 
-def _setup(seeds=(3, 11, 0, 7), geometry=None):
+def _setup(seeds=SEEDS, geometry=None, shapes=SHAPES):
     from tip import potential as P
     from tip.ensemble import Ensemble
-    ens = Ensemble(SHAPES, seeds=list(seeds))
+    ens = Ensemble(shapes, seeds=list(seeds))
     if geometry:
         ens.set_geometry(*geometry)
     rng = np.random.default_rng(1)
     init, tables, params = [], [], []
     p = P.ModelParams.real()
-    for i, s in enumerate(SHAPES):
+    for i, s in enumerate(shapes):
         phi = (rng.normal(size=s) * 2e3).astype(np.float32)
         pi = (rng.normal(size=s) * 3e2).astype(np.float32)
         ens.upload(i, "phi", phi)
@@ -52,7 +56,7 @@ def _same(a, b):
 @pytest.mark.parametrize("mode", ["sequential", "batched", "graph"])
 def test_modes_match_single_lattice(mode):
     from tip.lattice import Lattice
-    seeds = (3, 11, 0, 7)
+    seeds = SEEDS
     ens, init, tables, params = _setup(seeds)
     ens.step(6, mode=mode)
     got = _state(ens)
@@ -83,6 +87,35 @@ def test_geometries_identical(geometry):
     assert _same(_state(ens_b), ref)
     ens_a.close()
     ens_b.close()
+
+
+# every ny divisible by 4, so two and four y rows per thread apply
+SHAPES_TY = [(8, 16, 64), (6, 12, 96), (4, 8, 24), (6, 20, 70), (5, 8, 128)]
+
+
+@pytest.mark.parametrize("ty", [2, 4])
+@pytest.mark.parametrize("group_bytes", [0, 1])
+def test_row_mappings_several_rows_per_thread(ty, group_bytes):
+    """Row kernel with TY = 2 and 4 over dense rows (nz = 64, 128 in 256-thread blocks; 96, 24
+    in 192-thread blocks) and whole-wave rows (nz = 70), one group or one group per instance,
+    equal bit for bit to the single-lattice step."""
+    from tip.lattice import Lattice
+    seeds = (4, 8, 15, 16, 23)
+    ens, init, tables, params = _setup(seeds, geometry=(ty, 3, 1, 0, 1), shapes=SHAPES_TY)
+    ens.set_schedule(group_bytes)
+    ens.step(2, mode="batched")
+    ens.step(3, mode="graph")
+    got = _state(ens)
+    for i, s in enumerate(SHAPES_TY):
+        lat = Lattice(s, np.float32)
+        lat.upload("phi", init[i][0])
+        lat.upload("pi", init[i][1])
+        lat.set_table(*tables[i], odd=True)
+        lat.step(5, seed=seeds[i], **params[i])
+        assert np.array_equal(lat.download("phi"), got[i][0]), (ty, group_bytes, i)
+        assert np.array_equal(lat.download("pi"), got[i][1]), (ty, group_bytes, i)
+        lat.close()
+    ens.close()
 
 
 @pytest.mark.parametrize("group_bytes", [1, 40000, 1 << 30])

@@ -340,9 +340,11 @@ __device__ __forceinline__ unsigned rk2_real_tile_f32(
 // bounds the flat tile (its y/z neighbours are L1/L2 requests).  A block holds G groups of
 // consecutive row blocks (G = 4/W for narrow rows, so a block has 4 waves).  TY must divide
 // ny; groups past the last row are idle.  The next plane's centres and halo rows are
-// loaded one plane ahead.
+// loaded one plane ahead.  With DENSE (the ensemble's small lattices) a group is nz
+// consecutive threads rather than whole waves, so rows whose length is not a multiple of 64
+// leave no lanes idle when the block size is a multiple of nz.
 // ---------------------------------------------------------------------------------------
-template <bool ODD, int TY, int NM, bool CNT>
+template <bool ODD, int TY, int NM, bool CNT, bool DENSE = false>
 __device__ __forceinline__ unsigned rk2_real_rows_tile(
     const float* __restrict__ phi, const float* __restrict__ pi, float* __restrict__ phi_out,
     float* __restrict__ pi_out, const float* __restrict__ noise, const StepScalars<float>& c,
@@ -353,16 +355,30 @@ __device__ __forceinline__ unsigned rk2_real_rows_tile(
     // consecutive lanes (stride K words) hit distinct banks.  Every slot of a column is a
     // compile-time offset from one of three per-thread base addresses (z, z+1, z-1).
     constexpr int K = 4 * TY + 1;
-    const int G = (int)blockDim.y / W;
-    const int w = (int)threadIdx.y, grp = w / W, zw = w - grp * W;
-    const int z = zw * 64 + (int)threadIdx.x;
+    // Thread -> (row group, z).  Default: a group is W whole waves (z = 64 zw + lane).  DENSE:
+    // groups of nz threads packed back to back (W unused); the block's thread count must be
+    // a multiple of nz, so a row of 96 sites takes 96 lanes instead of two waves.
+    int G, grp, z, pitch;
+    if constexpr (DENSE) {
+        const int t = (int)threadIdx.y * 64 + (int)threadIdx.x;
+        G = ((int)blockDim.y * 64) / nz;
+        grp = t / nz;
+        z = t - grp * nz;
+        pitch = nz;
+    } else {
+        G = (int)blockDim.y / W;
+        const int w = (int)threadIdx.y;
+        grp = w / W;
+        z = (w - grp * W) * 64 + (int)threadIdx.x;
+        pitch = 64 * W;
+    }
     const bool zact = z < nz;
     const int zc = zact ? z : 0;
     const int zp = zc + 1 < nz ? zc + 1 : 0, zm = zc > 0 ? zc - 1 : nz - 1;
     const int y0 = (tile * G + grp) * TY;
     const bool act = zact && y0 < ny;
     const int yb = y0 < ny ? y0 : 0;
-    float* Lg = lds + (size_t)grp * (64 * W) * K;
+    float* Lg = lds + (size_t)grp * pitch * K;
     float* Lz = Lg + z * K;
     const float* Lzp = Lg + zp * K;
     const float* Lzm = Lg + zm * K;

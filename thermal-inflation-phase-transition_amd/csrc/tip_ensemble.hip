@@ -9,6 +9,8 @@
 // buffers sit in a device descriptor array, and the noise step counter is a device word, so a
 // K-step loop (K launches + one counter bump) is captured once into a hipGraph and replayed;
 // parameter updates rewrite the descriptors and leave the graph valid.
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <vector>
 
@@ -69,9 +71,11 @@ __global__ __launch_bounds__(64 * BY) void k_rk2_ensemble(
     }
 }
 
-// Row-tile variant (LDS staging, y rows per thread TY): a block is 4 waves; an instance with
-// W waves per z row gets 4/W row groups per block.
-template <int TY, bool ODD, bool CNT>
+// Row-tile variant (LDS staging, y rows per thread TY).  Default mapping: a block is 4 waves
+// and an instance with W waves per z row gets 4/W row groups per block.  DENSE: row groups of
+// nz threads packed back to back; the launch's block size (256 or 192 threads) is a multiple
+// of the nz of every instance it covers.
+template <int TY, bool ODD, bool CNT, bool DENSE>
 __global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
     const EnsInst* __restrict__ insts, const EnsItem* __restrict__ items, int nitems, int cx,
     int parity, const uint64_t* __restrict__ step_base, int step_off, int xcd_remap,
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
         tab.off = I.off;
         const int x1 = min(it.x0 + cx, I.nx);
         __syncthreads();  // the previous item's last plane may still be read from LDS
-        unsigned cnt = rk2_real_rows_tile<ODD, TY, NOISE_HASH, CNT>(
+        unsigned cnt = rk2_real_rows_tile<ODD, TY, NOISE_HASH, CNT, DENSE>(
             I.f[parity][0], I.f[parity][1], I.f[parity ^ 1][0], I.f[parity ^ 1][1], nullptr, I.c,
             tab, I.nx, I.ny, I.nz, I.W, it.tp, it.x0, x1, step ^ I.seed_mix, esc, lds_ens);
         if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(&vac[it.inst], (unsigned long long)cnt);
@@ -107,6 +111,12 @@ __global__ void k_advance(uint64_t* step_base, int n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) step_base[0] += (uint64_t)n;
 }
+
+// A run of work items launched together: items [first, last), threads per block, and whether
+// the dense row mapping applies.
+struct EnsGroup {
+    int first, last, threads, dense;
+};
 
 }  // namespace
 
@@ -133,10 +143,11 @@ struct tip_ensemble {
     // per thread, 0 = default), cx = x planes per work item
     int kind = TIP_KERNEL_AUTO, by = 0, cx = 8, xcd = 1, persistent = 0;
     int use_rows = 0, ty = 1, flat_by = 4;  // resolved by build_items
-    // schedule: instances are stepped in groups of at most group_bytes of field data (both
-    // buffers of phi and pi); each group runs all steps of a call before the next group
+    // schedule: instances are stepped in groups of about group_bytes of field data (both
+    // buffers of phi and pi), split evenly inside each block-size class; each group runs all
+    // steps of a call before the next group
     int64_t group_bytes = 0;
-    std::vector<std::pair<int, int>> groups;  // item ranges [first, last)
+    std::vector<EnsGroup> groups;
     EnsItem* dev_items = nullptr;
     int nitems = 0;
     bool items_dirty = true;
@@ -228,38 +239,77 @@ int resolve_kernel(tip_ensemble* E)
     return TIP_OK;
 }
 
+// Threads per block of the dense row mapping for a z extent: 256 or 192 when that is a
+// multiple of nz (rows of 96 sites: two per 3-wave block, where whole waves per row would
+// leave a quarter of the lanes idle), else 0 (whole waves per z row, 256 threads).
+int dense_threads(int64_t nz)
+{
+    if (256 % nz == 0) return 256;
+    if (192 % nz == 0) return 192;
+    return 0;
+}
+
 int build_items(tip_ensemble* E)
 {
     if (!E->items_dirty) return TIP_OK;
     int rc = resolve_kernel(E);
     if (rc) return rc;
+    // Instances ordered by block-size class (stable), so that each launch has one block size;
+    // the stepping order of independent instances does not change any result.
+    std::vector<int> order(E->n), cls(E->n, 0);
+    for (int i = 0; i < E->n; ++i) {
+        order[i] = i;
+        cls[i] = E->use_rows ? dense_threads(E->dims[3 * i + 2]) : 0;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a] < cls[b]; });
     std::vector<EnsItem> items;
     E->groups.clear();
-    int64_t gbytes = 0;
-    size_t gfirst = 0;
-    for (int i = 0; i < E->n; ++i) {
-        const int nx = (int)E->dims[3 * i], ny = (int)E->dims[3 * i + 1], nz = (int)E->dims[3 * i + 2];
-        const int64_t ibytes = (int64_t)inst_sites(E, i) * 16;
-        if (E->group_bytes > 0 && gbytes > 0 && gbytes + ibytes > E->group_bytes) {
-            E->groups.push_back({(int)gfirst, (int)items.size()});
-            gfirst = items.size();
-            gbytes = 0;
+    for (size_t k = 0; k < order.size();) {
+        const int cl = cls[order[k]];
+        size_t k1 = k;
+        double cbytes = 0.0;
+        for (; k1 < order.size() && cls[order[k1]] == cl; ++k1)
+            cbytes += 16.0 * (double)inst_sites(E, order[k1]);
+        const int threads = cl ? cl : 256;
+        // a class of B bytes becomes ceil(B / group_bytes) groups of about equal bytes
+        // (contiguous runs of its instances); no schedule: one group per class
+        int64_t ng = 1;
+        if (E->group_bytes > 0)
+            ng = std::min<int64_t>((int64_t)(k1 - k),
+                                   (int64_t)std::ceil(cbytes / (double)E->group_bytes));
+        ng = std::max<int64_t>(ng, 1);
+        double cum = 0.0;
+        int64_t gcur = 0;
+        size_t gfirst = items.size();
+        for (size_t j = k; j < k1; ++j) {
+            const int i = order[j];
+            const int nx = (int)E->dims[3 * i], ny = (int)E->dims[3 * i + 1];
+            const int nz = (int)E->dims[3 * i + 2];
+            const double ibytes = 16.0 * (double)inst_sites(E, i);
+            const int64_t gi =
+                std::min<int64_t>(ng - 1, (int64_t)((cum + 0.5 * ibytes) / cbytes * (double)ng));
+            if (gi != gcur && items.size() > gfirst) {
+                E->groups.push_back({(int)gfirst, (int)items.size(), threads, cl ? 1 : 0});
+                gfirst = items.size();
+            }
+            gcur = gi;
+            cum += ibytes;
+            int tiles;
+            if (E->use_rows) {
+                int W = (nz + 63) / 64;
+                if (W == 3) W = 4;
+                E->host_inst[i].W = W;
+                const int G = cl ? threads / nz : 4 / W;
+                tiles = (ny + G * E->ty - 1) / (G * E->ty);
+            } else {
+                tiles = (ny * nz + 64 * E->flat_by - 1) / (64 * E->flat_by);
+            }
+            for (int x0 = 0; x0 < nx; x0 += E->cx)
+                for (int t = 0; t < tiles; ++t) items.push_back({i, t, x0, 0});
         }
-        gbytes += ibytes;
-        int tiles;
-        if (E->use_rows) {
-            int W = (nz + 63) / 64;
-            if (W == 3) W = 4;
-            E->host_inst[i].W = W;
-            const int G = 4 / W;
-            tiles = (ny + G * E->ty - 1) / (G * E->ty);
-        } else {
-            tiles = (ny * nz + 64 * E->flat_by - 1) / (64 * E->flat_by);
-        }
-        for (int x0 = 0; x0 < nx; x0 += E->cx)
-            for (int t = 0; t < tiles; ++t) items.push_back({i, t, x0, 0});
+        E->groups.push_back({(int)gfirst, (int)items.size(), threads, cl ? 1 : 0});
+        k = k1;
     }
-    E->groups.push_back({(int)gfirst, (int)items.size()});
     E->desc_dirty = true;
     rc = sync_descriptors(E);
     if (rc) return rc;
@@ -276,25 +326,30 @@ int build_items(tip_ensemble* E)
     return TIP_OK;
 }
 
-int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long* vac, float esc,
-                   int item0, int nit)
+// One step of the items of group g.
+int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g)
 {
-    const EnsItem* items = E->dev_items + item0;
+    const EnsItem* items = E->dev_items + g.first;
+    const int nit = g.last - g.first;
+    unsigned long long* const vac = nullptr;
+    const float esc = 0.0f;
     int nb = nit;
     if (E->persistent > 0 && E->persistent < nb) nb = E->persistent;
     const int remap = (E->xcd && nb % 8 == 0) ? 1 : 0;
     if (E->use_rows) {
-        const size_t lds = (size_t)256 * (4 * E->ty + 1) * sizeof(float);
-#define TIP_LAUNCH_ENSR_C(TYV, ODD, CNT)                                                      \
-    hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD, CNT>), dim3(nb), dim3(64, 4), lds,       \
-                       E->stream, E->dev_inst, items, nit, E->cx, parity, E->dev_step,          \
-                       step_off, remap, vac, esc)
-#define TIP_LAUNCH_ENSR(TYV, ODD)                \
-    do {                                         \
-        if (vac)                                 \
-            TIP_LAUNCH_ENSR_C(TYV, ODD, true);   \
-        else                                     \
-            TIP_LAUNCH_ENSR_C(TYV, ODD, false);  \
+        const size_t lds = (size_t)g.threads * (4 * E->ty + 1) * sizeof(float);
+        TIP_REQUIRE(g.threads == 256 || (g.dense && g.threads == 192),
+                    "ensemble: bad block size %d", g.threads);
+#define TIP_LAUNCH_ENSR_C(TYV, ODD, DENSE)                                                     \
+    hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD, false, DENSE>), dim3(nb),              \
+                       dim3(64, g.threads / 64), lds, E->stream, E->dev_inst, items, nit, E->cx, \
+                       parity, E->dev_step, step_off, remap, vac, esc)
+#define TIP_LAUNCH_ENSR(TYV, ODD)                 \
+    do {                                          \
+        if (g.dense)                              \
+            TIP_LAUNCH_ENSR_C(TYV, ODD, true);    \
+        else                                      \
+            TIP_LAUNCH_ENSR_C(TYV, ODD, false);   \
     } while (0)
 #define TIP_LAUNCH_ENSR_TY(ODD)                        \
     switch (E->ty) {                                   \
@@ -338,19 +393,11 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, unsigned long long
 // All launches of nsteps steps starting at buffer parity par0 (no counter bump).
 int issue_steps(tip_ensemble* E, int64_t nsteps, int par0)
 {
-    if (E->groups.size() > 1) {
-        for (const auto& g : E->groups)
-            for (int64_t s = 0; s < nsteps; ++s) {
-                int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, nullptr, 0.0f, g.first,
-                                        g.second - g.first);
-                if (rc) return rc;
-            }
-        return TIP_OK;
-    }
-    for (int64_t s = 0; s < nsteps; ++s) {
-        int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, nullptr, 0.0f, 0, E->nitems);
-        if (rc) return rc;
-    }
+    for (const auto& g : E->groups)
+        for (int64_t s = 0; s < nsteps; ++s) {
+            int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, g);
+            if (rc) return rc;
+        }
     return TIP_OK;
 }
 

@@ -58,8 +58,9 @@ class Ensemble:
                   int(planes_per_block), int(bool(xcd_remap)), int(persistent_blocks))
 
     def set_schedule(self, group_bytes: int = 0) -> None:
-        """Step instances in groups of at most group_bytes of field data (16 B per site), each
-        group through all steps of a call before the next (0 = all together)."""
+        """Step instances in groups of about group_bytes of field data (16 B per site; each
+        block-size class of the row kernel cut into groups of equal bytes), each group through
+        all steps of a call before the next (0 = one group per class)."""
         _lib.call("tip_ensemble_set_schedule", self._h, int(group_bytes))
 
     def step(self, nsteps: int, mode: str = "graph", timed: bool = False):
