@@ -161,22 +161,20 @@ def bench_ensemble(steps: int = 200):
     for i, (t, tmin, dinv, prm) in enumerate(inst):
         ens.set_table(i, t, tmin, dinv, odd=True)
         ens.set_params(i, **prm)
-    ens.set_schedule(224 << 20)  # tune the geometry under a cache-sized schedule
-    tune = autotune.tune_ensemble(ens, steps=20, use_cache=False)
-    log(f"ensemble autotune best {tune['best']} {tune['us_per_step']:.1f} us/step")
-    sched = autotune.tune_ensemble_schedule(ens, steps=steps, use_cache=False)
-    log(f"ensemble schedule best {sched['best']} bytes/group {sched['us_per_step']:.1f} us/step "
-        f"({sched['table']})")
     rng = np.random.default_rng(0)
     for i, s in enumerate(shapes):
         ens.upload(i, "phi", 0.01 * rng.normal(size=s))
         ens.upload(i, "pi", np.zeros(s))
-    # every mode is timed from the same state, past the initial roll to the true vacuum
+    # every mode is timed from the same state, past the initial roll to the true vacuum; the
+    # launch geometry and the step schedule are tuned together on that state
     ens.step(300, mode="batched")
     state = [(ens.download(i, "phi"), ens.download(i, "pi")) for i in range(len(shapes))]
+    tune = autotune.tune_ensemble_joint(ens, steps=100, use_cache=False)
+    log(f"ensemble autotune best geometry {tune['best'][0]}, {tune['best'][1]} bytes/group: "
+        f"{tune['us_per_step']:.1f} us/step ({tune['table']})")
     sites = float(sum(int(np.prod(s)) for s in shapes))
-    out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"],
-           "group_bytes": sched["best"]}
+    out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"][0],
+           "group_bytes": tune["best"][1]}
     for mode in ("graph", "batched", "sequential"):
         for i, (phi, pi) in enumerate(state):
             ens.upload(i, "phi", phi)

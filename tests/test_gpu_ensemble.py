@@ -173,3 +173,28 @@ def test_autotune_picks_and_keeps_results(tmp_path, monkeypatch):
     assert (tmp_path / "cache.json").exists()
     rec2 = autotune.tune_ensemble(ens, candidates=[(8, 8, 1, 0)], steps=2)
     assert rec2["best"] == rec["best"]  # served from the cache
+
+
+def test_joint_tuner_picks_a_pair_and_keeps_results(tmp_path, monkeypatch):
+    from tip import autotune
+    monkeypatch.setattr(autotune, "CACHE_PATH", tmp_path / "cache.json")
+    geoms = [(1, 4, 1, 0, 1), (1, 8, 1, 0, 2), (1, 16, 1, 7, 1)]
+    ens, _, _, _ = _setup()
+    rec = autotune.tune_ensemble_joint(ens, geometries=geoms, group_sizes=(0, 1, 40000),
+                                       top=2, steps=2)
+    g, gb = rec["best"]
+    assert tuple(g) in geoms and gb in (0, 1, 40000)
+    assert len(rec["table"]) == 2 * 3 and len(rec["geometries"]) == 3
+    rec2 = autotune.tune_ensemble_joint(ens, geometries=[(1, 3, 0, 0, 1)],
+                                        group_sizes=(0, 1, 40000), top=2, steps=2)
+    assert rec2["best"] == rec["best"]  # served from the cache
+    ens.close()
+    # stepping under the tuned pair gives the untuned fields
+    ens_a, _, _, _ = _setup()
+    ens_a.step(4, mode="batched")
+    ens_b, _, _, _ = _setup(geometry=tuple(g))
+    ens_b.set_schedule(gb)
+    ens_b.step(4, mode="batched")
+    assert _same(_state(ens_b), _state(ens_a))
+    ens_a.close()
+    ens_b.close()
