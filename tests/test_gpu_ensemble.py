@@ -90,17 +90,18 @@ def test_geometries_identical(geometry):
 
 
 # every ny divisible by 4, so two and four y rows per thread apply
-SHAPES_TY = [(8, 16, 64), (6, 12, 96), (4, 8, 24), (6, 20, 70), (5, 8, 128)]
+SHAPES_TY = [(8, 16, 64), (6, 12, 96), (4, 8, 24), (6, 20, 70), (5, 8, 128), (3, 4, 192),
+             (2, 8, 200)]
 
 
 @pytest.mark.parametrize("ty", [2, 4])
 @pytest.mark.parametrize("group_bytes", [0, 1])
 def test_row_mappings_several_rows_per_thread(ty, group_bytes):
     """Row kernel with TY = 2 and 4 over dense rows (nz = 64, 128 in 256-thread blocks; 96, 24
-    in 192-thread blocks) and whole-wave rows (nz = 70), one group or one group per instance,
-    equal bit for bit to the single-lattice step."""
+    in 192-thread blocks, 192 filling one) and whole-wave rows (nz = 70, 200), one group or one
+    group per instance, equal bit for bit to the single-lattice step."""
     from tip.lattice import Lattice
-    seeds = (4, 8, 15, 16, 23)
+    seeds = (4, 8, 15, 16, 23, 42, 7)
     ens, init, tables, params = _setup(seeds, geometry=(ty, 3, 1, 0, 1), shapes=SHAPES_TY)
     ens.set_schedule(group_bytes)
     ens.step(2, mode="batched")
