@@ -104,6 +104,21 @@ int tip_rk2_step_table_f32(float *phi, float *pi, double dt, double dx, double e
                            int64_t nz, uint64_t noise_seed, double noise_scale,
                            unsigned long long *vac_count, double vac_esc, void *stream);
 
+/* Allocation-free variant of tip_rk2_step_table_f32 (same arguments, same results).
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The float32 segment table goes to a static device array of at most 65537 table points
+ * (512 KiB per device, held until the process ends) instead of library scratch; vp_npts
+ * above that returns TIP_ERR_ARG.  No hipMalloc/hipFree is called.  All calls on one device
+ * share that array: calls that overlap on one device race. */
+int tip_rk2_step_table_f32_noalloc(float *phi, float *pi, double dt, double dx,
+                                   double eta_eff, double mu, const float *noise,
+                                   float *phi_tmp, float *pi_tmp, double inv_a2,
+                                   const double *vp_table, double vp_tmin, double vp_dinv,
+                                   int64_t vp_npts, int vp_odd, int64_t nx, int64_t ny,
+                                   int64_t nz, uint64_t noise_seed, double noise_scale,
+                                   unsigned long long *vac_count, double vac_esc, void *stream);
+
 /* Out-of-place fused step (the performance form: ping-pong buffers, no copy back).
  * (phi, pi) -> (phi_out, pi_out); the input and output buffers must not overlap.
  * table_ab is a float32 segment table from tip_table_prepare_f32 (npts-1 float2 entries). */

@@ -1,5 +1,7 @@
 // tip_step_f32.hip -- float32 fused RK2 step (the performance path), its launch geometry and
 // the float32 segment table it reads.
+#include <mutex>
+
 #include "step_kernels.h"
 
 using namespace tip;
@@ -434,5 +436,76 @@ extern "C" int tip_rk2_step_table_c32(
     TIP_HIP(hipMemcpyAsync(phi2, phi2_tmp, bytes, hipMemcpyDeviceToDevice, st));
     TIP_HIP(hipMemcpyAsync(pi1, pi1_tmp, bytes, hipMemcpyDeviceToDevice, st));
     TIP_HIP(hipMemcpyAsync(pi2, pi2_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    return TIP_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free form of tip_rk2_step_table_f32.  The float32 segment table lives in a
+// file-scope device array instead of library scratch grown with hipMalloc, so the call
+// allocates nothing.  Each device holds its own copy of the array at its own address; the
+// address is resolved once per device with hipGetSymbolAddress and only that address is
+// passed to kernels and copies.  Every call on one device shares the array: calls that
+// overlap on one device (several host threads or streams) race.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kNoAllocMaxPoints = 65537;  // table points: 65536 segments, 512 KiB
+static __device__ __align__(256) float2 scratch[kNoAllocMaxPoints - 1];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float2* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+int noalloc_scratch(float2** out)
+{
+    int dev = 0;
+    TIP_HIP(hipGetDevice(&dev));
+    TIP_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "no-alloc step: device %d out of range",
+                dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        TIP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TIP_REQUIRE(p != nullptr, "no-alloc step: scratch symbol has no device address");
+        g_noalloc_ptr[dev] = reinterpret_cast<float2*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return TIP_OK;
+}
+
+}  // namespace
+
+extern "C" int tip_rk2_step_table_f32_noalloc(float* phi, float* pi, double dt, double dx,
+                                              double eta_eff, double mu, const float* noise,
+                                              float* phi_tmp, float* pi_tmp, double inv_a2,
+                                              const double* vp_table, double vp_tmin,
+                                              double vp_dinv, int64_t vp_npts, int vp_odd,
+                                              int64_t nx, int64_t ny, int64_t nz,
+                                              uint64_t noise_seed, double noise_scale,
+                                              unsigned long long* vac_count, double vac_esc,
+                                              void* stream)
+{
+    hipStream_t st = as_stream(stream);
+    TIP_REQUIRE(phi_tmp && pi_tmp, "rk2_step_table_f32_noalloc: phi_tmp/pi_tmp are required");
+    TIP_REQUIRE(vp_table, "rk2_step_table_f32_noalloc: NULL table");
+    int rc = check_npts(vp_npts);
+    if (rc) return rc;
+    TIP_REQUIRE(vp_npts <= kNoAllocMaxPoints,
+                "rk2_step_table_f32_noalloc: vp_npts = %lld exceeds the %lld points of its "
+                "static table scratch", (long long)vp_npts, (long long)kNoAllocMaxPoints);
+    float2* ab = nullptr;
+    rc = noalloc_scratch(&ab);
+    if (rc) return rc;
+    rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
+    if (rc) return rc;
+    rc = launch_real_f32(phi, pi, phi_tmp, pi_tmp,
+                         make_scalars<float>(dt, dx, eta_eff, mu, inv_a2, noise_scale), noise,
+                         make_table32(ab, vp_tmin, vp_dinv, vp_npts, vp_odd), nx, ny, nz,
+                         noise_seed, (noise == nullptr && noise_scale != 0.0) ? 1 : 0, vac_count,
+                         (float)vac_esc, nullptr, st);
+    if (rc) return rc;
+    size_t bytes = (size_t)(nx * ny * nz) * sizeof(float);
+    TIP_HIP(hipMemcpyAsync(phi, phi_tmp, bytes, hipMemcpyDeviceToDevice, st));
+    TIP_HIP(hipMemcpyAsync(pi, pi_tmp, bytes, hipMemcpyDeviceToDevice, st));
     return TIP_OK;
 }

@@ -65,6 +65,9 @@ SIGNATURES = {
     "tip_rk2_step_table_f32": (_int, [
         _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _d, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
         _u64, _d, _vp, _d, _vp]),
+    "tip_rk2_step_table_f32_noalloc": (_int, [
+        _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _d, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
+        _u64, _d, _vp, _d, _vp]),
     "tip_rk2_fused_f64": (_int, [
         _vp, _vp, _vp, _vp, _d, _d, _d, _d, _d, _vp, _vp, _d, _d, _i64, _int, _i64, _i64, _i64,
         _u64, _d, _vp, _d, _vp]),
