@@ -199,3 +199,40 @@ def test_joint_tuner_picks_a_pair_and_keeps_results(tmp_path, monkeypatch):
     assert _same(_state(ens_b), _state(ens_a))
     ens_a.close()
     ens_b.close()
+
+
+def test_graph_after_table_param_and_geometry_changes():
+    """Graph replays stay equal to batched launches bit for bit when tables (including a switch
+    from odd to even tables, which changes the captured kernel variant), step parameters and
+    the launch geometry change between calls, and for calls longer than one captured chunk."""
+    from tip import potential as P
+    ens_b, _, _, params = _setup()
+    ens_g, _, _, _ = _setup()
+    p = P.ModelParams.real()
+
+    def both(fn):
+        fn(ens_b)
+        fn(ens_g)
+
+    def step(k):
+        ens_b.step(k, mode="batched")
+        ens_g.step(k, mode="graph")
+        assert _same(_state(ens_g), _state(ens_b)), k
+
+    step(4)
+    t, tmin, dinv = P.build_vprime_table(7300.0, 0.0, 1e5, 2048, p)
+    both(lambda e: e.set_table(1, t, tmin, dinv, odd=True))
+    prm = dict(params[2])
+    prm["eta_eff"] = 9.5
+    both(lambda e: e.set_params(2, **prm))
+    step(4)
+    # every instance to an even (signed) table: the odd/even kernel variant changes
+    te, temin, tedinv = P.build_vprime_table(7000.0, -1e5, 1e5, 4097, p)
+    for i in range(len(SHAPES)):
+        both(lambda e: e.set_table(i, te, temin, tedinv, odd=False))
+    step(4)
+    both(lambda e: e.set_geometry(1, 5, 1, 0, 1))
+    step(37)  # chunks of 16, 16 and 5 steps
+    assert ens_g.step_count == ens_b.step_count == 49
+    ens_b.close()
+    ens_g.close()

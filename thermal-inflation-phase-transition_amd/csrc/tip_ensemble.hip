@@ -315,6 +315,9 @@ int build_items(tip_ensemble* E)
     if (rc) return rc;
     TIP_REQUIRE(!items.empty() && items.size() < (size_t(1) << 30), "ensemble: bad work list");
     TIP_HIP(hipStreamSynchronize(E->stream));
+    // cached graphs hold the old work list's address in their kernel nodes: destroy them
+    // before that list is freed
+    drop_graphs(E);
     if (E->dev_items) TIP_HIP(hipFree(E->dev_items));
     E->dev_items = nullptr;
     TIP_HIP(hipMalloc(&E->dev_items, sizeof(EnsItem) * items.size()));
@@ -322,7 +325,6 @@ int build_items(tip_ensemble* E)
                       hipMemcpyHostToDevice));
     E->nitems = (int)items.size();
     E->items_dirty = false;
-    drop_graphs(E);
     return TIP_OK;
 }
 
@@ -412,6 +414,37 @@ int ready(tip_ensemble* E)
     int rc = sync_descriptors(E);
     if (rc) return rc;
     return build_items(E);
+}
+
+// Steps per captured graph: bounds a graph's node count (groups x steps + 1) whatever the
+// length of the call; an even count leaves the buffer parity unchanged, so one graph per
+// parity serves every full chunk.
+constexpr int64_t kGraphChunk = 16;
+
+// The instantiated graph of k steps starting at buffer parity `par` (captured on first use).
+int graph_for(tip_ensemble* E, int64_t k, int par, hipGraphExec_t* out)
+{
+    auto key = std::make_pair(k, par);
+    auto itg = E->graphs.find(key);
+    if (itg == E->graphs.end()) {
+        hipGraph_t g = nullptr;
+        TIP_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+        int rc = issue_steps(E, k, par);
+        if (rc) {
+            (void)hipStreamEndCapture(E->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)k);
+        TIP_HIP(hipStreamEndCapture(E->stream, &g));
+        hipGraphExec_t ex;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate");
+        itg = E->graphs.emplace(key, ex).first;
+    }
+    *out = itg->second;
+    return TIP_OK;
 }
 
 }  // namespace
@@ -510,6 +543,9 @@ extern "C" int tip_ensemble_set_table(tip_ensemble* E, int i, const double* host
     if (rc) return rc;
     TIP_HIP(hipStreamSynchronize(E->stream));
     EnsInst& I = E->host_inst[i];
+    // A captured graph fixes the kernel's odd/even table variant (and the stream is idle
+    // here): a change of that flag invalidates every cached graph.
+    if (E->has_table[i] && I.odd != (odd ? 1 : 0)) drop_graphs(E);
     I.ab = E->table_ab[i];
     I.tmin = (float)tmin;
     I.dinv = (float)dinv;
@@ -613,28 +649,20 @@ extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, floa
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
         TIP_HIP(hipGetLastError());
     } else {
-        TIP_REQUIRE(nsteps <= 100000, "tip_ensemble_step: graph of more than 1e5 steps");
-        auto key = std::make_pair(nsteps, E->parity);
-        auto itg = E->graphs.find(key);
-        if (itg == E->graphs.end()) {
-            hipGraph_t g;
-            TIP_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-            rc = issue_steps(E, nsteps, E->parity);
-            if (rc) {
-                (void)hipStreamEndCapture(E->stream, &g);
-                return rc;
-            }
-            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step,
-                               (int)nsteps);
-            TIP_HIP(hipStreamEndCapture(E->stream, &g));
-            hipGraphExec_t ex;
-            hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate");
-            itg = E->graphs.emplace(key, ex).first;
+        // The graph holds a chunk of at most kGraphChunk steps (groups x chunk kernel nodes
+        // plus the counter bump) and is replayed; a call of n steps is n / chunk replays of
+        // the full chunk and one of the remainder.  Each replay advances the device step
+        // counter by its own chunk, so the noise seeds equal those of batched launches.
+        int64_t left = nsteps;
+        while (left > 0) {
+            const int64_t k = left < kGraphChunk ? left : kGraphChunk;
+            hipGraphExec_t ex = nullptr;
+            rc = graph_for(E, k, E->parity, &ex);
+            if (rc) return rc;
+            TIP_HIP(hipGraphLaunch(ex, E->stream));
+            if (k & 1) E->parity ^= 1;
+            left -= k;
         }
-        TIP_HIP(hipGraphLaunch(itg->second, E->stream));
-        if (nsteps & 1) E->parity ^= 1;
     }
     E->step += (uint64_t)nsteps;
     if (elapsed_ms) {
