@@ -80,6 +80,8 @@ def lib():
         L.orc_langevin_c64.argtypes = [
             _int, _f64p, _f64p, _nf64, _nf64, _nf64, _nf64, _nf64, _nf64, _d, _d, _d, _d, _d, _d,
             _f64p, _d, _d, _i64, _d, _d, _d, _u64, _d, _i64, _i64, _i64]
+        L.orc_complex_force.argtypes = [_f64p, _f64p, _i64, _f64p, _d, _d, _i64, _d, _d, _d,
+                                        _f64p, _f64p]
         L.orc_cl_hash_ics.argtypes = [_f64p, _nf64, _i64, _d]
         L.orc_winding.argtypes = [_f64p, _f64p, _nf32, _ni8, _i64, _i64, _i64]
         L.orc_string_count.argtypes = [_f32p, _i64]
@@ -198,6 +200,18 @@ def rk2_step_table(phi, pi, dt, dx, eta_eff, mu, noise, phi_tmp, pi_tmp, inv_a2,
                                           nx, ny, nz)
     else:
         raise TypeError(phi.dtype)
+
+
+def complex_force(p1, p2, table, tmin, dinv, zn=(0.0, 0.0, 0.0)):
+    """(dV1, dV2) of the complex step's force at points (p1, p2): V'(rho) phi_i / rho from
+    the table plus the Z_N term (zn = (N, delta_V, active))."""
+    p1 = np.ascontiguousarray(p1, dtype=np.float64).ravel()
+    p2 = np.ascontiguousarray(p2, dtype=np.float64).ravel()
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    a, b = np.empty_like(p1), np.empty_like(p1)
+    lib().orc_complex_force(p1, p2, p1.size, t, tmin, dinv, t.size, float(zn[0]), float(zn[1]),
+                            float(zn[2]), a, b)
+    return a, b
 
 
 def rk2_step_table_complex(phi1, phi2, pi1, pi2, dt, dx, eta_eff, mu, noise1, noise2,

@@ -277,6 +277,21 @@ static inline void complex_force(double p1, double p2, const double *table, doub
     *dV2 = b;
 }
 
+/* The complex-field force (dV1, dV2) of complex_force at n points, exported so that its
+ * Z_N term can be pinned on its own (finite differences of the potential, the reference's
+ * C++ statement of the same force). */
+ORC_API void orc_complex_force(const double *p1, const double *p2, int64_t n,
+                               const double *table, double tmin, double dinv, int64_t npts,
+                               double zn_n, double zn_dv, double zn_active, double *dV1,
+                               double *dV2)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double rho = sqrt(p1[i] * p1[i] + p2[i] * p2[i]);
+        complex_force(p1[i], p2[i], table, tmin, dinv, npts, zn_n, zn_dv, zn_active, rho,
+                      &dV1[i], &dV2[i]);
+    }
+}
+
 ORC_API void orc_rk2_step_table_c64(double *phi1, double *phi2, double *pi1, double *pi2,
                                     double dt, double dx, double eta_eff, double mu,
                                     const double *noise1, const double *noise2, double *phi1_tmp,
