@@ -337,6 +337,29 @@ int tip_rho_max_c32(const float *phi1, const float *phi2, int64_t n, double *out
 #define TIP_REAL 0
 #define TIP_COMPLEX 1
 
+/* ---- V' table producer ----------------------------------------------------------------
+ * Replaces: build_vprime_table(T, phi_min, phi_max, n) -> (table, tmin, dinv) with
+ *           Vprime_scalar(phi, T), simulation/latticeSimeRescale_numba.py:805-849 (complex
+ *           driver simulation/latticeSimComplex_numba.py:818-862).
+ * table (device, n float64) [i] = V'(phi_i, T), phi_i = numpy.linspace(phi_min, phi_max, n)[i]:
+ * V' = lam phi^3 - mphi^2 phi + T^4/(2 pi^2) (coeff_b dJ_b(u_b) du_b/dphi
+ *      + coeff_f dJ_f(u_f) du_f/dphi), the dJ splines evaluated with the reference's clamp and
+ * Horner rule.  cb, cf (device): the splines' piecewise-cubic coefficients, scipy
+ * CubicSpline.c layout (4, nseg) row-major, rows = powers 3, 2, 1, 0, on knots x_min + i h
+ * (:432-455; the lattice counterterm, :758-795, is folded into them by the caller).
+ * coeff_b, coeff_f are _POT_COEFFS (convention x multiplicity, :231-241).  float64 arithmetic
+ * in the reference's operation order.  tmin, dinv (host, nullable) receive the reference's
+ * return values phi_min and 1 / ((phi_max - phi_min) / (n - 1)). */
+typedef struct tip_vprime_model {
+    double lam, mphi, mb2, yb, gb, yf, gf, coeff_b, coeff_f;
+    double x_min, h; /* spline knots x_min + i h, i = 0..nseg */
+    int64_t nseg;
+} tip_vprime_model;
+
+int tip_build_vprime_table(double *table, int64_t n, double T, double phi_min, double phi_max,
+                           const tip_vprime_model *model, const double *cb, const double *cf,
+                           double *tmin, double *dinv, void *stream);
+
 typedef struct tip_step_params {
     double dt, dx, eta_eff, mu, inv_a2;
     double noise_scale; /* 0 disables the inline hash noise */
@@ -353,6 +376,13 @@ int tip_lattice_upload(tip_lattice *lat, int which, const void *host);
 int tip_lattice_download(tip_lattice *lat, int which, void *host);
 int tip_lattice_set_table(tip_lattice *lat, const double *host_table, int64_t npts,
                           double tmin, double dinv, int odd);
+/* Build the lattice's V' table on the device (tip_build_vprime_table into the lattice's own
+ * table buffer, then the float32 segments): no host evaluation and no upload.  Replaces the
+ * host build + set_table of the driver's rebuild rule (latticeSimeRescale_numba.py:3650-3672);
+ * the lattice records tmin = phi_min, dinv = (npts - 1) / (phi_max - phi_min). */
+int tip_lattice_build_table(tip_lattice *lat, int64_t npts, double T, double phi_min,
+                            double phi_max, const tip_vprime_model *model, const double *cb,
+                            const double *cf, int odd);
 int tip_lattice_set_zn(tip_lattice *lat, double zn_n, double zn_dv, double zn_active);
 /* Launch geometry of the float32 step kernels (the autotuner's knobs).  kernel:
  * TIP_KERNEL_AUTO (row tiles where they apply, else flat tiles), TIP_KERNEL_ROWS (whole z

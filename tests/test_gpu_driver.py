@@ -2,9 +2,9 @@
 surrogate path's multi-step calls equal one call per step.
 
 Bars:
-  * real float64 with Hubble cooling, 30 steps at 32^3, tables rebuilt by the driver's rule:
-    fields within 1e-12 relative of the oracle fed the same tables, schedule and noise seeds;
-    true-vacuum series exact
+  * real float64 with Hubble cooling, 30 steps at 32^3, tables rebuilt by the driver's rule on
+    the device (no host table build): fields within 1e-12 relative of the oracle fed the host
+    producer's tables for the same requests, schedule and noise seeds; true-vacuum series exact
   * complex float32 with the surrogate under linear cooling: one call per report interval
     (noise amplitude scaled by sqrt(T_n / T_start) inside) within 1e-5 relative of one call per
     step (noise amplitude from the formula at every step)
@@ -19,26 +19,34 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_driver_real_f64_replays_on_oracle():
+def test_driver_real_f64_replays_on_oracle(monkeypatch):
     from tip import driver as D
     from tip import potential as P
     cfg = D.RunConfig(field="real", Nx=32, Ny=32, Nz=32, Nt=30, T0=7000.0, precision="f64",
                       no_hubble=False, report=10, seed=3, quiet=True, table_size=4096)
     run = D.Run(cfg)
-    tables = []  # (step index the table is first used at, table, tmin, dinv)
-    orig = run.lat.set_table
+    builds = []  # (step index the table is first used at, T, phi_min, phi_max, npts)
+    orig = run.lat.build_table
 
-    def rec(table, tmin, dinv, odd=False):
-        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
-        orig(table, tmin, dinv, odd=odd)
+    def rec(T, lo, hi, n, model, spline=None, odd=False):
+        builds.append((run.lat.step_count, T, lo, hi, n))
+        return orig(T, lo, hi, n, model, spline, odd=odd)
 
-    run.lat.set_table = rec
+    def host_build(*a, **k):
+        raise AssertionError("the driver built a V' table on the host")
+
+    run.lat.build_table = rec
+    monkeypatch.setattr(P, "build_vprime_table", host_build)
     try:
         series = run.run()
         phi_g, pi_g = run.lat.download("phi"), run.lat.download("pi")
     finally:
         run.close()
-    assert len(tables) >= 1 and series["step"] == [10, 20, 30]
+        monkeypatch.undo()
+    assert len(builds) >= 1 and series["step"] == [10, 20, 30]
+    # the oracle is fed the host producer's tables of the same (T, range) requests
+    tables = [(n0, *P.build_vprime_table(T, lo, hi, n, run.model, run.spline))
+              for n0, T, lo, hi, n in builds]
 
     f0 = D.initial_fields(cfg)
     phi, pi = f0["phi"].copy(), f0["pi"].copy()
@@ -151,19 +159,21 @@ def test_driver_real_f64_other_integrators_replay(integrator):
                       no_hubble=False, report=8, seed=4, quiet=True, table_size=4096,
                       integrator=integrator)
     run = D.Run(cfg)
-    tables = []
-    orig = run.lat.set_table
+    builds = []
+    orig = run.lat.build_table
 
-    def rec(table, tmin, dinv, odd=False):
-        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
-        orig(table, tmin, dinv, odd=odd)
+    def rec(T, lo, hi, n, model, spline=None, odd=False):
+        builds.append((run.lat.step_count, T, lo, hi, n))
+        return orig(T, lo, hi, n, model, spline, odd=odd)
 
-    run.lat.set_table = rec
+    run.lat.build_table = rec
     try:
         run.run()
         phi_g, pi_g = run.lat.download("phi"), run.lat.download("pi")
     finally:
         run.close()
+    tables = [(n0, *P.build_vprime_table(T, lo, hi, n, run.model, run.spline))
+              for n0, T, lo, hi, n in builds]
     f0 = D.initial_fields(cfg)
     phi, pi = f0["phi"].copy(), f0["pi"].copy()
     hub = P.HubbleState(cfg.T0, D.MU, cfg.eta_phys / D.MU)

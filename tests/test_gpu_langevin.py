@@ -246,20 +246,23 @@ def test_driver_complex_integrators_replay_on_oracle(integrator):
     cfg = D.RunConfig(field="complex", Nx=16, Ny=16, Nz=64, Nt=12, T0=7350.0, cool_to=7000.0,
                       report=6, seed=0, quiet=True, init_rho=300.0, integrator=integrator,
                       table_size=4096, zn_order=3, zn_strength=1e8)
+    from tip import potential as P
     run = D.Run(cfg)
-    tables = []
-    orig = run.lat.set_table
+    builds = []  # the driver builds its tables on the device; replay them from the host's
+    orig = run.lat.build_table
 
-    def rec(table, tmin, dinv, odd=False):
-        tables.append((run.lat.step_count, np.array(table), tmin, dinv))
-        orig(table, tmin, dinv, odd=odd)
+    def rec(T, lo, hi, n, model, spline=None, odd=False):
+        builds.append((run.lat.step_count, T, lo, hi, n))
+        return orig(T, lo, hi, n, model, spline, odd=odd)
 
-    run.lat.set_table = rec
+    run.lat.build_table = rec
     try:
         run.run()
         got = [run.lat.download(k).astype(np.float64) for k in ("phi1", "phi2", "pi1", "pi2")]
     finally:
         run.close()
+    tables = [(n0, *P.build_vprime_table(T, lo, hi, n, run.model, run.spline))
+              for n0, T, lo, hi, n in builds]
     f0 = D.initial_fields(cfg)
     a, b = f0["phi1"].astype(np.float64), f0["phi2"].astype(np.float64)
     u, v = f0["pi1"].astype(np.float64), f0["pi2"].astype(np.float64)

@@ -44,6 +44,7 @@ COMMON = [
 PER_FILE = {
     "tip_step_f64.hip": ["-ffp-contract=off"],
     "tip_langevin_f64.hip": ["-ffp-contract=off"],
+    "tip_potential.hip": ["-ffp-contract=off"],
 }
 
 

@@ -270,6 +270,13 @@ struct SegTable {
 int prepare_table_f32(const double* table_dev, int64_t npts, double tmin, double dinv,
                       float2* ab_dev, hipStream_t s);
 
+// The V' table producer on the device (tip_potential.hip): table[i] = V'(phi_i, T) on
+// numpy.linspace(phi_min, phi_max, n); table_grid gives the reference's (tmin, dinv).
+int build_vprime_table_dev(double* table, int64_t n, double T, double phi_min, double phi_max,
+                           const tip_vprime_model& m, const double* cb, const double* cf,
+                           hipStream_t s);
+void table_grid(int64_t n, double phi_min, double phi_max, double* tmin, double* dinv);
+
 // Lookup on the host of one segment table built from a host table (used for argument checks).
 int check_table(const void* table, int64_t npts, double dinv);
 

@@ -187,6 +187,42 @@ extern "C" int tip_lattice_set_table(tip_lattice* lat, const double* host_table,
     return TIP_OK;
 }
 
+extern "C" int tip_lattice_build_table(tip_lattice* lat, int64_t npts, double T, double phi_min,
+                                       double phi_max, const tip_vprime_model* model,
+                                       const double* cb, const double* cf, int odd)
+{
+    TIP_REQUIRE(lat && model && cb && cf, "tip_lattice_build_table: NULL");
+    TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30),
+                "tip_lattice_build_table: npts out of range");
+    TIP_REQUIRE(!(lat->kind == TIP_COMPLEX && odd), "complex tables are tabulated on rho >= 0");
+    double tmin, dinv;
+    table_grid(npts, phi_min, phi_max, &tmin, &dinv);
+    TIP_REQUIRE(dinv > 0.0 && dinv < 1e300, "tip_lattice_build_table: empty table range");
+    int rc = close_baoab(lat);  // the deferred half-kick belongs to the old table
+    if (rc) return rc;
+    if (npts > lat->cap) {
+        TIP_HIP(hipStreamSynchronize(lat->stream));
+        if (lat->table) TIP_HIP(hipFree(lat->table));
+        if (lat->table_ab) TIP_HIP(hipFree(lat->table_ab));
+        lat->table = nullptr;
+        lat->table_ab = nullptr;
+        TIP_HIP(hipMalloc(&lat->table, npts * sizeof(double)));
+        TIP_HIP(hipMalloc(&lat->table_ab, (npts - 1) * sizeof(float2)));
+        lat->cap = npts;
+    }
+    // stream-ordered after every step already issued on the lattice: no host sync needed
+    rc = build_vprime_table_dev(lat->table, npts, T, phi_min, phi_max, *model, cb, cf,
+                                lat->stream);
+    if (rc) return rc;
+    rc = prepare_table_f32(lat->table, npts, tmin, dinv, lat->table_ab, lat->stream);
+    if (rc) return rc;
+    lat->npts = npts;
+    lat->tmin = tmin;
+    lat->dinv = dinv;
+    lat->odd = odd ? 1 : 0;
+    return TIP_OK;
+}
+
 extern "C" int tip_lattice_set_zn(tip_lattice* lat, double zn_n, double zn_dv, double zn_active)
 {
     TIP_REQUIRE(lat, "tip_lattice_set_zn: NULL");

@@ -31,6 +31,15 @@ class StepParams(C.Structure):
 _ppv = C.POINTER(C.c_void_p)
 
 
+class VprimeModel(C.Structure):
+    """tip_vprime_model of include/tip.h."""
+
+    _fields_ = [("lam", C.c_double), ("mphi", C.c_double), ("mb2", C.c_double),
+                ("yb", C.c_double), ("gb", C.c_double), ("yf", C.c_double), ("gf", C.c_double),
+                ("coeff_b", C.c_double), ("coeff_f", C.c_double), ("x_min", C.c_double),
+                ("h", C.c_double), ("nseg", C.c_int64)]
+
+
 class MlpFeatures(C.Structure):
     """tip_mlp_features of include/tip.h."""
 
@@ -100,6 +109,8 @@ SIGNATURES = {
     "tip_lattice_upload": (_int, [_vp, _int, _vp]),
     "tip_lattice_download": (_int, [_vp, _int, _vp]),
     "tip_lattice_set_table": (_int, [_vp, _vp, _i64, _d, _d, _int]),
+    "tip_lattice_build_table": (_int, [_vp, _i64, _d, _d, _d, _vp, _vp, _vp, _int]),
+    "tip_build_vprime_table": (_int, [_vp, _i64, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tip_lattice_set_zn": (_int, [_vp, _d, _d, _d]),
     "tip_lattice_set_geometry": (_int, [_vp, _int, _int, _int, _int]),
     "tip_lattice_step": (_int, [_vp, C.POINTER(StepParams), _i64, _u64, _u64,

@@ -199,10 +199,11 @@ class Run:
             grid = np.linspace(t_lo, t_hi, self.cfg.table_size, dtype=np.float64)
             table = P.vprime_full(grid, T, self.model, include_cw=True)
             tmin, dinv = float(t_lo), 1.0 / ((t_hi - t_lo) / (self.cfg.table_size - 1))
+            self.lat.set_table(table, tmin, dinv, odd=False)
         else:
-            table, tmin, dinv = P.build_vprime_table(T, t_lo, t_hi, self.cfg.table_size,
-                                                     self.model, self.spline)
-        self.lat.set_table(table, tmin, dinv, odd=False)
+            # the reference's table producer (Vprime_scalar over the grid) on the device,
+            # straight into the lattice's table buffer
+            self.lat.build_table(T, t_lo, t_hi, self.cfg.table_size, self.model, self.spline)
         self.table_T, self.table_lo, self.table_hi = T, t_lo, t_hi
         self.rebuilds += 1
 

@@ -65,6 +65,22 @@ class Lattice:
                   float(dinv), int(bool(odd)))
         self.table_info = (float(t[0]), float(tmin), float(dinv), t.size, bool(odd))
 
+    def build_table(self, T: float, phi_min: float, phi_max: float, npts: int, model,
+                    spline=None, odd: bool = False) -> tuple:
+        """Build the V' table of (T, model) on the device, straight into the lattice's table
+        buffer (tip_lattice_build_table; no host evaluation, no upload).  Returns (tmin,
+        dinv) as the reference's build_vprime_table does."""
+        from . import potential as P
+        from . import vprime_device as VD
+        sp = spline or P.dj_spline()
+        cb, cf = VD.spline_on_device(sp)
+        m = VD.model_struct(model, sp)
+        _lib.call("tip_lattice_build_table", self._h, int(npts), float(T), float(phi_min),
+                  float(phi_max), C.byref(m), cb.ptr, cf.ptr, int(bool(odd)))
+        tmin, dinv = float(phi_min), 1.0 / ((float(phi_max) - float(phi_min)) / (int(npts) - 1))
+        self.table_info = (None, tmin, dinv, int(npts), bool(odd))
+        return tmin, dinv
+
     def set_zn(self, n: float, dv: float, active: float) -> None:
         _lib.call("tip_lattice_set_zn", self._h, float(n), float(dv), float(active))
 
