@@ -53,18 +53,44 @@ def real_workload(n: int = 256, npts: int = 8192):
     return table, tmin, dinv, prm
 
 
-def traffic_from_profiles(name: str):
-    """HBM bytes per launch recorded by the committed rocprofv3 --pmc summary, if present."""
+def profile_record(name: str) -> dict:
+    """The `key: value` lines of a committed rocprofv3 summary under profiles/ (HBM bytes per
+    launch from FETCH_SIZE / WRITE_SIZE, the kernel-trace average of the same workload)."""
     f = ROOT / "profiles" / name
+    out = {}
     if not f.exists():
-        return None
+        return out
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if ":" in line and not line.startswith(("#", "|", " ")):
+            k, v = line.split(":", 1)
             try:
-                return float(line.split(":", 1)[1].strip())
+                out[k.strip()] = float(v.strip())
             except ValueError:
-                return None
-    return None
+                pass
+    return out
+
+
+def host_description() -> dict:
+    """CPU model, SMT state and core counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = Path("/sys/devices/system/cpu/smt/active").read_text().strip() == "1"
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "smt_active": smt, "nproc": avail,
+            "os_cpu_count": os.cpu_count()}
 
 
 def cpu_baseline(shape, table, tmin, dinv, prm, budget_s: float = 12.0):
@@ -98,6 +124,43 @@ def cpu_baseline(shape, table, tmin, dinv, prm, budget_s: float = 12.0):
                         f"rk2_step_table (2 OpenMP passes, float32 fields, float64 arithmetic) "
                         f"+ hash noise pass, {el:.1f} s"),
                 ms_per_step=1e3 * el / steps)
+
+
+def cpu_baseline_ensemble(budget_s: float = 6.0) -> dict:
+    """The nucleation ensemble (64 realisations, 32 x 64^3 + 24 x 96^3 + 8 x 128^3, per-instance
+    tables) stepped as a sequential loop on the host cores: one oracle float32 rk2_step_table
+    (+ its hash noise pass) per instance per step."""
+    from oracle import oracle as O
+
+    shapes, seeds, _, inst = ensemble_workload()
+    rng = np.random.default_rng(0)
+    fields = []
+    for s in shapes:
+        phi = (0.01 * rng.normal(size=s)).astype(np.float32)
+        fields.append((phi, np.zeros(s, np.float32), np.empty(s, np.float32),
+                       np.empty(s, np.float32), np.empty(s, np.float32)))
+
+    def one(n):
+        for (phi, pi, pt, qt, noise), sd, (t, tmin, dinv, prm) in zip(fields, seeds, inst):
+            O.lib().orc_noise_field_f32(noise, None, *phi.shape, O.noise_seed(n, sd), 73856093,
+                                        prm["noise_scale"])
+            O.rk2_step_table(phi, pi, prm["dt"], prm["dx"], prm["eta_eff"], prm["mu"], noise, pt,
+                             qt, prm["inv_a2"], t, tmin, dinv, len(t), odd=True)
+
+    one(0)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        one(steps + 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 50:
+            break
+    sites = float(sum(int(np.prod(s)) for s in shapes))
+    return dict(value=sites * steps / el, unit="site-updates/s", cores=O.num_threads(),
+                kind="port", ms_per_step=1e3 * el / steps,
+                sample=(f"{steps} ensemble steps (64 realisations, {int(sites)} sites) as a "
+                        f"sequential per-instance loop of the oracle's float32 rk2_step_table "
+                        f"+ hash noise pass, {el:.1f} s"))
 
 
 def cpu_baseline_integrators(table, tmin, dinv, prm, n: int = 128, reps: int = 3) -> dict:
@@ -152,7 +215,7 @@ def ensemble_workload(npts: int = 8192, seed: int = 2024):
     return shapes, list(range(1, len(shapes) + 1)), T0s, inst
 
 
-def bench_ensemble(steps: int = 200):
+def bench_ensemble(steps: int = 200, modes=("graph", "batched", "sequential")):
     from tip import autotune
     from tip.ensemble import Ensemble
 
@@ -175,7 +238,7 @@ def bench_ensemble(steps: int = 200):
     sites = float(sum(int(np.prod(s)) for s in shapes))
     out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"][0],
            "group_bytes": tune["best"][1]}
-    for mode in ("graph", "batched", "sequential"):
+    for mode in modes:
         for i, (phi, pi) in enumerate(state):
             ens.upload(i, "phi", phi)
             ens.upload(i, "pi", pi)
@@ -186,14 +249,20 @@ def bench_ensemble(steps: int = 200):
         out[mode] = {"site_updates_per_s": sites * steps / wall, "ms_per_step": 1e3 * wall / steps,
                      "event_ms_per_step": ms / steps}
         log(f"ensemble {mode}: {1e3 * wall / steps:.3f} ms/step")
-    seq = out["sequential"]["site_updates_per_s"]
-    out["graph_vs_sequential"] = out["graph"]["site_updates_per_s"] / seq
-    out["batched_vs_sequential"] = out["batched"]["site_updates_per_s"] / seq
-    # the grouped ensemble's rate: the faster of its two launch forms (graph replay or direct
-    # launches of the same grouped kernels) against the plain per-instance loop
-    out["grouped_vs_sequential"] = max(out["graph_vs_sequential"], out["batched_vs_sequential"])
-    out["hbm_gbps_graph"] = 16.0 * out["graph"]["site_updates_per_s"] / 1e9
-    out["hbm_gbps_batched"] = 16.0 * out["batched"]["site_updates_per_s"] / 1e9
+    if "sequential" in out:
+        seq = out["sequential"]["site_updates_per_s"]
+        ratios = []
+        for mode in ("graph", "batched"):
+            if mode in out:
+                out[f"{mode}_vs_sequential"] = out[mode]["site_updates_per_s"] / seq
+                ratios.append(out[f"{mode}_vs_sequential"])
+        # the grouped ensemble's rate: the faster of its two launch forms (graph replay or
+        # direct launches of the same grouped kernels) against the plain per-instance loop
+        if ratios:
+            out["grouped_vs_sequential"] = max(ratios)
+    for mode in ("graph", "batched"):
+        if mode in out:
+            out[f"hbm_gbps_{mode}"] = 16.0 * out[mode]["site_updates_per_s"] / 1e9
     ens.close()
     return out
 
@@ -383,6 +452,8 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the complex, float64 and surrogate throughput lines")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ensemble-modes", default="graph,batched,sequential",
+                    help="comma-separated subset of graph, batched, sequential")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, 5 device-to-device copies of 128 MiB (a known-size "
                          "reference for the HBM byte counters of a profiler)")
@@ -429,7 +500,8 @@ def main() -> int:
     kern_s = ev_ms * 1e-3 / a.steps
     achieved = sites * bps / kern_s / 1e9
     prof_name = "rk2_fused_f32_256cube.md" if dtype == np.float32 else "rk2_fused_f64_256cube.md"
-    traffic = traffic_from_profiles(prof_name)
+    prof = profile_record(prof_name)
+    traffic = prof.get("traffic_bytes_per_launch")
     out = {
         "metric": "site-updates/s + HBM GB/s at 256^3 FP32; batched ensemble vs plain loop",
         "value": value,
@@ -450,7 +522,12 @@ def main() -> int:
         "hbm_gbps_algorithmic": sites * bps * a.steps / wall / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel_us": kern_s * 1e6, "bytes_per_site": bps},
+                     "kernel_us": kern_s * 1e6, "bytes_per_site": bps,
+                     # traffic is not measured in this run: it is the committed rocprofv3
+                     # FETCH_SIZE + WRITE_SIZE record of this workload, whose own kernel
+                     # average is traffic_kernel_us
+                     "traffic_source": f"profiles/{prof_name}" if traffic else None,
+                     "traffic_kernel_us": prof.get("kernel_average_us")},
         "true_vacuum_fraction": frac_true,
     }
     lat.close()
@@ -468,12 +545,21 @@ def main() -> int:
         dst.free()
     if not a.no_ensemble:
         log("batched ensemble (64 realisations, 46.4M sites) ...")
-        out["ensemble"] = bench_ensemble()
+        out["ensemble"] = bench_ensemble(modes=[m for m in a.ensemble_modes.split(",") if m])
     if not a.no_extras:
         out["extras"] = bench_extras()
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)
+        out["cpu_baseline"]["host"] = host_description()
+        t64, tmin64, dinv64, prm64 = real_workload(64)
+        leg = cpu_baseline((64, 64, 64), t64, tmin64, dinv64, prm64, a.cpu_seconds / 4)
+        out["cpu_baseline"]["legs"] = {"real64_f32": leg}
+        log(f"cpu 64^3: {leg['ms_per_step']:.2f} ms/step")
+        if not a.no_ensemble:
+            leg = cpu_baseline_ensemble(a.cpu_seconds / 2)
+            out["cpu_baseline"]["legs"]["nucleation_ensemble_sequential"] = leg
+            log(f"cpu ensemble: {leg['ms_per_step']:.1f} ms/step")
         if "extras" in out:
             out["extras"]["cpu_baseline_integrators"] = cpu_baseline_integrators(
                 table, tmin, dinv, prm)

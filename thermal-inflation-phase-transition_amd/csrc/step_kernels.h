@@ -340,7 +340,7 @@ __device__ __forceinline__ unsigned rk2_real_tile_f32(
 // bounds the flat tile (its y/z neighbours are L1/L2 requests).  A block holds G groups of
 // consecutive row blocks (G = 4/W for narrow rows, so a block has 4 waves).  TY must divide
 // ny; groups past the last row are idle.  The next plane's centres and halo rows are
-// loaded one plane ahead.  With DENSE (the ensemble's small lattices) a group is nz
+// loaded one plane ahead (not on the chunk's last plane).  With DENSE (the ensemble's small lattices) a group is nz
 // consecutive threads rather than whole waves, so rows whose length is not a multiple of 64
 // leave no lanes idle when the block size is a multiple of nz.
 // ---------------------------------------------------------------------------------------
@@ -409,7 +409,8 @@ __device__ __forceinline__ unsigned rk2_real_rows_tile(
 
     // Centre window x-1, x, x+1 and the slot for x+2 (w0..w3 change roles each plane), and
     // halo rows (y0-1, y0+TY) of the current / next plane.
-    float p0[TY], q0[TY], p1[TY], q1[TY], p2[TY], q2[TY], p3[TY], q3[TY];
+    float p0[TY], q0[TY], p1[TY], q1[TY], p2[TY], q2[TY];
+    float p3[TY] = {}, q3[TY] = {};  // zero where a chunk's last plane loads nothing
     {
         const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
         const int xq = x0 + 1 < nx ? x0 + 1 : 0;
@@ -420,7 +421,7 @@ __device__ __forceinline__ unsigned rk2_real_rows_tile(
             p2[r] = buf_load<float>(at(P, xq), off[r]); q2[r] = buf_load<float>(at(Q, xq), off[r]);
         }
     }
-    float hA[4], hB[4];  // phi y-1, pi y-1, phi y+TY, pi y+TY
+    float hA[4], hB[4] = {};  // phi y-1, pi y-1, phi y+TY, pi y+TY
     hA[0] = buf_load<float>(at(P, x0), offm); hA[1] = buf_load<float>(at(Q, x0), offm);
     hA[2] = buf_load<float>(at(P, x0), offp); hA[3] = buf_load<float>(at(Q, x0), offp);
     unsigned cnt = 0;
@@ -436,7 +437,9 @@ __device__ __forceinline__ unsigned rk2_real_rows_tile(
             Lz[oP + r] = pc[r];
             Lz[oQ + r] = qc[r];
         }
-        {
+        // the next plane's centres (x + 2) and halo rows (x + 1); the chunk's last plane has
+        // no next plane (a wave-uniform branch): those planes belong to the next chunk
+        if (x + 1 < x1) {
             const int xa = x + 1 < nx ? x + 1 : 0;
             const int xb = xa + 1 < nx ? xa + 1 : 0;
 #pragma unroll
@@ -926,7 +929,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const uint64_t rowK = (uint64_t)(uint32_t)nz * (uint64_t)kNoiseMult;
     uint64_t xK = (uint64_t)(uint32_t)x0 * planeK;
 
-    float v0[NF][TY], v1[NF][TY], v2[NF][TY], v3[NF][TY];
+    float v0[NF][TY], v1[NF][TY], v2[NF][TY], v3[NF][TY] = {};
     {
         const int xm = x0 - 1 >= 0 ? x0 - 1 : nx - 1;
         const int xq = x0 + 1 < nx ? x0 + 1 : 0;
@@ -939,7 +942,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 v2[f][r] = buf_load_s(at(F[f], xq), vo, so[r]);
             }
     }
-    float hA[2 * NF], hB[2 * NF];  // [f] row y0-1, [NF + f] row y0+TY
+    float hA[2 * NF], hB[2 * NF] = {};  // [f] row y0-1, [NF + f] row y0+TY
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         hA[f] = buf_load_s(at(F[f], x0), vo, som);
@@ -970,7 +973,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int r = 0; r < TY; ++r) Lz[(b * NF + f) * TY + r] = vc[f][r];
-        {
+        if (x + 1 < x1) {  // the chunk's last plane has no next plane
             const int xa = x + 1 < nx ? x + 1 : 0;
             const int xb = xa + 1 < nx ? xa + 1 : 0;
 #pragma unroll
