@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/tip.h"
+#include "bm_math.h"
 
 namespace tip {
 
@@ -79,15 +80,18 @@ __host__ __device__ __forceinline__ uint64_t site_seed(int64_t site, uint64_t no
     return (uint64_t)(site * kNoiseMult) ^ noise_seed;
 }
 
-// float64 Box-Muller exactly as written in the reference (cos for the first component,
-// sin for the second of the complex pair, latticeSimComplex_numba.py:2965-2974).
+// float64 Box-Muller as written in the reference (cos for the first component, sin for the
+// second of the complex pair, latticeSimComplex_numba.py:2965-2974), with the log and the
+// cos / sin of bm_math.h on the draw's bounded domains (within a few ulp of libm).
 __device__ __forceinline__ double gauss_f64(uint64_t seed)
 {
     uint64_t x1, x2;
     hash_states(seed, x1, x2);
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
-    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    double s, c;
+    bm_sincos(6.283185307179586 * u2, &s, &c);
+    return sqrt(-2.0 * bm_log(u1)) * c;
 }
 
 __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double& zs)
@@ -96,10 +100,9 @@ __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double
     hash_states(seed, x1, x2);
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
-    double r = sqrt(-2.0 * log(u1));
-    double a = 6.283185307179586 * u2;
+    double r = sqrt(-2.0 * bm_log(u1));
     double s, c;
-    sincos(a, &s, &c);
+    bm_sincos(6.283185307179586 * u2, &s, &c);
     zc = r * c;
     zs = r * s;
 }
