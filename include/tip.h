@@ -479,14 +479,18 @@ int tip_ensemble_set_step(tip_ensemble *ens, uint64_t step);
  * by 1e-4, :3650-3672).  A 2 -> 32 -> 32 -> 1 tanh MLP, float32, parameter vector of
  * TIP_MLP_NPARAMS floats laid out W1[32][2], b1[32], W2[32][32], b2[32], w3[32], b3.
  * Inputs: features x = (f1, f2), f1 = asinh(rho/rho_scale)/asinh(rho_max/rho_scale),
- * f2 = (T - T_mid)/T_half.  V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * y * e(rho)),
- * e = 1/(1 + (rho/rho_cut)^2): the network carries the thermal + Coleman-Weinberg part of
- * V'/rho over a fixed envelope, the tree term is exact. */
+ * f2 = (T - T_mid)/T_half.
+ * V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * e(rho) * (g0(f2) + f1 * y)),
+ * e = 1/(1 + (rho/rho_cut)^2), g0(f2) = g0[0] + g0[1] f2 + g0[2] f2^2 + g0[3] f2^3: the tree
+ * term is exact, g0 is the thermal part of V'/rho at rho -> 0 (the effective mass, fitted to
+ * the float64 potential; f1 = 0 there), and the network carries the rest of the thermal +
+ * Coleman-Weinberg part of V'/rho over a fixed envelope. */
 #define TIP_MLP_HIDDEN 32
 #define TIP_MLP_NPARAMS 1185
 
 typedef struct tip_mlp_features {
     double rho_scale, rho_max, T_mid, T_half, out_scale, lam, m2, rho_cut;
+    double g0[4];
 } tip_mlp_features;
 
 /* y[i] = MLP(x[2i], x[2i+1]) */

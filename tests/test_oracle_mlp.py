@@ -46,16 +46,28 @@ def test_surrogate_target_is_thermal_part():
     spec = S.SurrogateSpec()
     rho = np.array([1e-1, 1e2, 1e4, 1e6, 1e9, 1e11])
     T = np.full_like(rho, 7000.0)
-    g = spec.g_target(rho, T)
+    g = spec.g_full(rho, T)
     # Boltzmann suppression: nothing thermal left far above T
     assert abs(g[3]) < 1e-10 and abs(g[4]) < 1e-10
-    # rho -> 0: g -> (1/m^2) T^2/(2 pi^2) sum_s c_s dJ_s(u_s)/u_s * y_s^2/2 (chain rule limit)
     p = spec.model
-    ub = math.sqrt(p.mb2 + (0.25 * p.yb ** 2 + 2 / 3 * p.gb ** 2) * 7000.0 ** 2) / 7000.0
-    uf = math.sqrt((1 / 6) * p.gf ** 2 * 7000.0 ** 2) / 7000.0
-    lim = 7000.0 ** 2 / (2 * math.pi ** 2) * (p.coeff_b * P.dJb(ub)[0] / ub * 0.5 * p.yb ** 2
-                                             + p.coeff_f * P.dJf(uf)[0] / uf * 0.5 * p.yf ** 2)
-    assert abs(g[0] - lim / p.mphi ** 2) < 1e-6 * abs(lim / p.mphi ** 2)
+
+    def lim(Tv):
+        """rho -> 0: g -> (1/m^2) T^2/(2 pi^2) sum_s c_s dJ_s(u_s)/u_s * y_s^2/2 (the chain-rule
+        limit of the thermal part; the Coleman-Weinberg part vanishes as rho^2 log rho)."""
+        ub = math.sqrt(p.mb2 + (0.25 * p.yb ** 2 + 2 / 3 * p.gb ** 2) * Tv ** 2) / Tv
+        uf = math.sqrt((1 / 6) * p.gf ** 2 * Tv ** 2) / Tv
+        return Tv ** 2 / (2 * math.pi ** 2) * (
+            p.coeff_b * P.dJb(ub)[0] / ub * 0.5 * p.yb ** 2
+            + p.coeff_f * P.dJf(uf)[0] / uf * 0.5 * p.yf ** 2) / p.mphi ** 2
+
+    assert abs(g[0] - lim(7000.0)) < 1e-6 * abs(lim(7000.0))
+    # the effective-mass fit g0(T) the kernels use is that limit across the range
+    for Tv in (6500.0, 6770.0, 7000.0, 7350.0, 7500.0):
+        assert abs(spec.g0_of(Tv) - lim(Tv)) < 1e-7, Tv
+    # the network's target is the rest of g over f1
+    y = spec.g_target(rho, T)
+    f1 = spec.x(rho, T)[:, 0]
+    np.testing.assert_allclose(spec.g0_of(T) + f1 * y, g, rtol=1e-12, atol=1e-12)
 
 
 def test_features_and_sampling():

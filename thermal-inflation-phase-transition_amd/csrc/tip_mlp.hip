@@ -5,10 +5,12 @@
 // Parameter vector (1185 float32): W1[32][2], b1[32], W2[32][32], b2[32], w3[32], b3.
 // Features: f1 = asinh(rho / rho_scale) / asinh(rho_max / rho_scale)  (resolves the thermal
 // barrier, a few T wide, and the vacuum at ~1e11 GeV in one input), f2 = (T - T_mid) / T_half.
-// Output: V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * y / (1 + (rho/rho_cut)^2)): the
-// network carries only the thermal and Coleman-Weinberg part of V'/rho (smooth,
-// Boltzmann-suppressed at large rho) divided by a fixed envelope, so its residual in the flat
-// large-rho region is suppressed; the tree term is exact.
+// Output: V'(rho, T) = rho * (lam rho^2 - m2 + out_scale * (g0(f2) + f1 y) * e(rho)),
+// e = 1 / (1 + (rho/rho_cut)^2): the tree term is exact; g0(f2), a cubic in f2, is the thermal part of V'/rho at rho -> 0
+// (the effective mass, fitted on the host to the float64 potential), so the effective mass
+// does not depend on the network (f1 = 0 at rho = 0); the network carries the rest of the
+// thermal and Coleman-Weinberg part of V'/rho (smooth, Boltzmann-suppressed at large rho)
+// divided by a fixed envelope, so its residual in the flat large-rho region is suppressed.
 //
 // Training is deterministic: each 64-sample block writes its partial gradient and one kernel
 // sums the partials in block order and applies Adam.
@@ -34,15 +36,21 @@ __device__ __forceinline__ float tanh_f(float x)
 
 struct FeatureMap {
     float inv_rho_s, inv_norm, T_mid, inv_T_half, out_scale, inv_rho_cut;
+    float g0c[4];
     double lam, m2;
     // The tree term lam rho^3 - m2 rho cancels near the vacuum; it is formed in float64 (full
-    // rate on MI355X) so V' is rounded to float32 once, relative to its own size.
-    __device__ __forceinline__ float vprime(float rho, float y) const
+    // rate on MI355X) so V' is rounded to float32 once, relative to its own size.  b is the
+    // bracket g0(f2) + f1 y.
+    __device__ __forceinline__ float vprime(float rho, float b) const
     {
         const float q = rho * inv_rho_cut;
-        const float th = out_scale * y / fmaf(q, q, 1.0f);
+        const float th = out_scale * b / fmaf(q, q, 1.0f);
         const double r = (double)rho;
         return (float)(r * (fma(lam * r, r, -m2) + (double)th));
+    }
+    __host__ __device__ __forceinline__ float g0(float f2v) const
+    {
+        return fmaf(fmaf(fmaf(g0c[3], f2v, g0c[2]), f2v, g0c[1]), f2v, g0c[0]);
     }
     __device__ __forceinline__ float f1(float rho) const
     {
@@ -63,6 +71,7 @@ FeatureMap make_fm(const tip_mlp_features* fm)
     m.lam = fm->lam;
     m.m2 = fm->m2;
     m.inv_rho_cut = (float)(1.0 / fm->rho_cut);
+    for (int k = 0; k < 4; ++k) m.g0c[k] = (float)fm->g0[k];
     return m;
 }
 
@@ -95,10 +104,12 @@ __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, float x0,
 struct MlpForce {
     const float* P;
     FeatureMap fm;
-    float f2;  // temperature feature of this launch
+    float f2;   // temperature feature of this launch
+    float g0v;  // g0(f2)
     __device__ __forceinline__ float operator()(float rho) const
     {
-        return fm.vprime(rho, mlp_eval(P, fm.f1(rho), f2));
+        const float x1 = fm.f1(rho);
+        return fm.vprime(rho, fmaf(x1, mlp_eval(P, x1, f2), g0v));
     }
 };
 
@@ -227,7 +238,8 @@ __global__ void k_mlp_vprime(const float* __restrict__ P, FeatureMap fm,
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float r = rho[s];
-    vp[s] = fm.vprime(r, mlp_eval(P, fm.f1(r), fm.f2(T[s])));
+    const float x1 = fm.f1(r), x2 = fm.f2(T[s]);
+    vp[s] = fm.vprime(r, fmaf(x1, mlp_eval(P, x1, x2), fm.g0(x2)));
 }
 
 __global__ void k_mlp_tabulate(const float* __restrict__ P, FeatureMap fm, float f2, double x0,
@@ -237,7 +249,8 @@ __global__ void k_mlp_tabulate(const float* __restrict__ P, FeatureMap fm, float
     if (i >= n) return;
     const float r = (float)(x0 + h * (double)i);
     const float a = fabsf(r);
-    const float v = fm.vprime(a, mlp_eval(P, fm.f1(a), f2));
+    const float x1 = fm.f1(a);
+    const float v = fm.vprime(a, fmaf(x1, mlp_eval(P, x1, f2), fm.g0(f2)));
     table[i] = (double)(r < 0.0f ? -v : v);
 }
 
@@ -332,15 +345,16 @@ extern "C" int tip_mlp_tabulate(const float* params, const tip_mlp_features* fm,
 
 namespace {
 
-// Grid of the network at one temperature: segment i of y(g) on g in [i, i+1], g the grid
-// coordinate of f1 = i * f1_step.
-__global__ void k_mlp_grid(const float* __restrict__ P, float f2, int nseg, float f1_step,
-                           float2* __restrict__ seg)
+// Grid of the network at one temperature: segment i of the bracket b(g) = g0 + f1 y on g in
+// [i, i+1], g the grid coordinate of f1 = i * f1_step.
+__global__ void k_mlp_grid(const float* __restrict__ P, float f2, float g0v, int nseg,
+                           float f1_step, float2* __restrict__ seg)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nseg) return;
-    const float y0 = mlp_eval(P, (float)i * f1_step, f2);
-    const float y1 = mlp_eval(P, (float)(i + 1) * f1_step, f2);
+    const float a0 = (float)i * f1_step, a1 = (float)(i + 1) * f1_step;
+    const float y0 = fmaf(a0, mlp_eval(P, a0, f2), g0v);
+    const float y1 = fmaf(a1, mlp_eval(P, a1, f2), g0v);
     seg[i] = make_float2(y0, y1 - y0);
 }
 
@@ -374,8 +388,10 @@ int launch_complex_mlp_grid(const float* phi1, const float* phi2, const float* p
     const FeatureMap f = make_fm(fm);
     const int nseg = (int)(npts - 1);
     const float f1_step = kGridF1Max / (float)nseg;
-    hipLaunchKernelGGL(k_mlp_grid, dim3((nseg + 255) / 256), dim3(256), 0, st, mlp_params,
-                       (float)((T - fm->T_mid) / fm->T_half), nseg, f1_step, grid);
+    const float f2 = (float)((T - fm->T_mid) / fm->T_half);
+    const float g0v = (float)(fm->g0[0] + f2 * (fm->g0[1] + f2 * (fm->g0[2] + f2 * fm->g0[3])));
+    hipLaunchKernelGGL(k_mlp_grid, dim3((nseg + 255) / 256), dim3(256), 0, st, mlp_params, f2,
+                       g0v, nseg, f1_step, grid);
     TIP_HIP(hipGetLastError());
     MlpGridRadial rad;
     rad.seg = grid;
@@ -439,6 +455,7 @@ int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, c
     f.P = mlp_params;
     f.fm = make_fm(fm);
     f.f2 = (float)((T - fm->T_mid) / fm->T_half);
+    f.g0v = f.fm.g0(f.f2);
     StepGeometry g;
     TileGrid tg;
     int nb;

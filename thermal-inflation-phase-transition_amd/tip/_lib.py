@@ -45,7 +45,7 @@ class MlpFeatures(C.Structure):
 
     _fields_ = [("rho_scale", C.c_double), ("rho_max", C.c_double), ("T_mid", C.c_double),
                 ("T_half", C.c_double), ("out_scale", C.c_double), ("lam", C.c_double),
-                ("m2", C.c_double), ("rho_cut", C.c_double)]
+                ("m2", C.c_double), ("rho_cut", C.c_double), ("g0", C.c_double * 4)]
 
 
 # name -> (restype, argtypes); every symbol of include/tip.h

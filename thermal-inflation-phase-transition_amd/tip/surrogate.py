@@ -3,11 +3,19 @@ re-tabulation and the inline-MLP complex step.
 
 Ground truth is the float64 finite-temperature potential with the Coleman-Weinberg term
 (tip.potential.vprime_full, the assembly of cosmolattice_ext/models/thermal_tables.hpp:96-146
-with exact Bessel-series thermal functions).  The network (2 -> 32 -> 32 -> 1 tanh, float32)
-learns g(rho, T) = (V'/rho - (lam rho^2 - m^2)) / (out_scale e(rho)), the thermal + CW part of
-V'/rho over the fixed envelope e = 1/(1 + (rho/rho_cut)^2); the tree term and the envelope are
-applied exactly in the kernels, so the network's residual far above the thermal scale (where
-the true thermal part is exponentially small) cannot leak into V' near the vacuum.
+with exact Bessel-series thermal functions).  With g(rho, T) = (V'/rho - (lam rho^2 - m^2)) /
+(out_scale e(rho)) the thermal + CW part of V'/rho over the fixed envelope
+e = 1/(1 + (rho/rho_cut)^2), the kernels assemble
+
+    V'/rho = lam rho^2 - m^2 + out_scale e(rho) (g0(f2) + f1 y),
+
+where g0(f2) is a cubic in the temperature feature fitted to g(0, T) (the effective mass
+M_eff^2 = V'/rho at rho -> 0, which decides the spinodal instability near T_c and which the
+network alone resolved only to ~5e-4 m^2) and y is the network (2 -> 32 -> 32 -> 1 tanh,
+float32), which learns (g - g0) / f1.  f1 = 0 at rho = 0, so the effective mass is the fit's,
+independent of the network; the tree term and the envelope are exact, so the network's
+residual far above the thermal scale (where the true thermal part is exponentially small)
+cannot leak into V' near the vacuum.
 
 Sampling: the |phi| range [0, 1.5 phi_true] is ~1e11 GeV wide while the thermal barrier lives
 at ~1e3 GeV, so half of the samples are uniform in |phi|/phi_true (the specified range) and
@@ -53,11 +61,31 @@ class SurrogateSpec:
         self.phi_true = P.thermal_vev(self.T_ref, self.model)
         self.rho_max = self.rho_frac * self.phi_true
         self.out_scale = self.model.mphi ** 2
+        self.g0 = self.fit_g0()
+
+    # rho at which V'/rho is taken as its rho -> 0 limit (its rho^2 term is < 1e-12 there)
+    RHO_ZERO = 1e-2
+
+    def fit_g0(self) -> np.ndarray:
+        """Cubic in f2 through g(0, T) over the training range widened by 10 % on each side
+        (least squares on 801 points; residual ~1e-8, against M_eff^2 / m^2 ~ 1e-3 at the
+        top of the range)."""
+        w = 0.1 * (self.T_hi - self.T_lo)
+        T = np.linspace(self.T_lo - w, self.T_hi + w, 801)
+        g = self.g_full(np.full_like(T, self.RHO_ZERO), T)
+        f2 = (T - 0.5 * (self.T_lo + self.T_hi)) / (0.5 * (self.T_hi - self.T_lo))
+        return np.polynomial.polynomial.polyfit(f2, g, 3)
+
+    def g0_of(self, T) -> np.ndarray:
+        f2 = (np.asarray(T, dtype=np.float64) - 0.5 * (self.T_lo + self.T_hi)) / (
+            0.5 * (self.T_hi - self.T_lo))
+        return np.polynomial.polynomial.polyval(f2, self.g0)
 
     def features(self) -> MlpFeatures:
         return MlpFeatures(self.rho_scale, self.rho_max, 0.5 * (self.T_lo + self.T_hi),
                            0.5 * (self.T_hi - self.T_lo), self.out_scale, self.model.lam,
-                           self.model.mphi ** 2, self.rho_cut)
+                           self.model.mphi ** 2, self.rho_cut,
+                           (C.c_double * 4)(*[float(c) for c in self.g0]))
 
     def x(self, rho, T) -> np.ndarray:
         rho, T = np.broadcast_arrays(np.asarray(rho, dtype=np.float64),
@@ -69,18 +97,31 @@ class SurrogateSpec:
     def envelope(self, rho) -> np.ndarray:
         return 1.0 / (1.0 + (np.asarray(rho, dtype=np.float64) / self.rho_cut) ** 2)
 
-    def g_target(self, rho, T) -> np.ndarray:
-        """(V'/rho - (lam rho^2 - m^2)) / (out_scale e(rho)) in float64."""
+    def g_full(self, rho, T) -> np.ndarray:
+        """g = (V'/rho - (lam rho^2 - m^2)) / (out_scale e(rho)) in float64."""
         rho = np.asarray(rho, dtype=np.float64)
         vp = P.vprime_full(rho, T, self.model, include_cw=self.include_cw)
         tree = self.model.lam * rho ** 2 - self.model.mphi ** 2
         return (vp / rho - tree) / (self.out_scale * self.envelope(rho))
 
+    def g_target(self, rho, T) -> np.ndarray:
+        """The network's target (g - g0(f2)) / f1 in float64."""
+        f1 = self.x(rho, T)[..., 0]
+        return (self.g_full(rho, T) - self.g0_of(T)) / f1
+
     def vprime_from_net(self, rho, T, y) -> np.ndarray:
         """V' assembled from network outputs y (float64), as the kernels assemble it."""
         rho = np.asarray(rho, dtype=np.float64)
+        f1 = self.x(rho, T)[..., 0]
+        b = self.g0_of(T) + f1 * np.asarray(y, dtype=np.float64)
         return rho * (self.model.lam * rho ** 2 - self.model.mphi ** 2
-                      + self.out_scale * np.asarray(y) * self.envelope(rho))
+                      + self.out_scale * b * self.envelope(rho))
+
+    def meff2(self, T) -> np.ndarray:
+        """The effective mass V'/rho at rho -> 0 of the float64 potential, in GeV^2."""
+        T = np.asarray(T, dtype=np.float64)
+        return P.vprime_full(np.full_like(T, self.RHO_ZERO), T, self.model,
+                             include_cw=self.include_cw) / self.RHO_ZERO
 
     def vprime_truth(self, rho, T) -> np.ndarray:
         return P.vprime_full(rho, T, self.model, include_cw=self.include_cw)
@@ -91,15 +132,16 @@ class SurrogateSpec:
 
     def weights(self, rho, T, barrier_tol: float = 1e-3, floored_tol: float = 1e-4):
         """Per-sample loss weights 1/tolerance^2 on the network output: an error dy moves
-        V' by rho m^2 e(rho) dy, which the floored metric divides by max(|V'|, floor) (tolerance
-        floored_tol) and the barrier metric by rho m^2 (tolerance barrier_tol).  Normalised to
-        mean 1."""
+        V' by rho m^2 e(rho) f1 dy, which the floored metric divides by max(|V'|, floor)
+        (tolerance floored_tol) and the barrier metric (V'/rho in units of m^2, which is what
+        the spinodal and the barrier near T_c depend on) by rho m^2 (tolerance barrier_tol).
+        Normalised to mean 1."""
         rho = np.asarray(rho, dtype=np.float64)
         T = np.asarray(T, dtype=np.float64)
-        env = self.envelope(rho)
+        k = self.envelope(rho) * self.x(rho, T)[..., 0]
         den = np.maximum(np.abs(self.vprime_truth(rho, T)), self.floor(T))
-        k_floor = rho * self.out_scale * env / den
-        w = np.maximum((k_floor / floored_tol) ** 2, (env / barrier_tol) ** 2)
+        k_floor = rho * self.out_scale * k / den
+        w = np.maximum((k_floor / floored_tol) ** 2, (k / barrier_tol) ** 2)
         return w / w.mean()
 
     def sample(self, n: int, seed: int = 0, T_margin: float = 0.0):
@@ -224,7 +266,7 @@ class Surrogate:
         np.savez(path, params=self.numpy_params(), rho_scale=sp.rho_scale, rho_cut=sp.rho_cut,
                  rho_frac=sp.rho_frac, T_lo=sp.T_lo, T_hi=sp.T_hi, T_ref=sp.T_ref,
                  model=json.dumps(dataclasses.asdict(sp.model)), include_cw=sp.include_cw,
-                 meta=json.dumps(meta))
+                 form=FORM, g0=sp.g0, meta=json.dumps(meta))
 
     @classmethod
     def load(cls, path=None) -> "Surrogate":
@@ -256,21 +298,28 @@ TRAINED_COOLING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data
                                "surrogate_cooling_6500_7500.npz")
 
 
+# Output form of the saved networks: V'/rho = tree + out_scale e (g0 + f1 y)
+FORM = "g0-cubic+f1-residual"
+
+
 def load_params_npz(path: str):
     """(SurrogateSpec, float32 parameters) of a saved network, without touching the GPU."""
     d = np.load(path, allow_pickle=False)
+    if "form" not in d.files or str(d["form"]) != FORM:
+        raise ValueError(f"{path}: network saved in another output form (expected {FORM})")
     model = P.ModelParams(**json.loads(str(d["model"])))
     spec = SurrogateSpec(model=model, T_lo=float(d["T_lo"]), T_hi=float(d["T_hi"]),
                          T_ref=float(d["T_ref"]), rho_frac=float(d["rho_frac"]),
                          rho_scale=float(d["rho_scale"]), rho_cut=float(d["rho_cut"]),
                          include_cw=bool(d["include_cw"]))
+    spec.g0 = np.asarray(d["g0"], np.float64)  # the fit the network was trained against
     return spec, np.asarray(d["params"], np.float32)
 
 
-def train_cooling_surrogate(epochs: int = 400, n_samples: int = 1 << 20, seed: int = 0, log=None):
-    """The recipe for the 7350 -> 6770 K cooling runs: complex-field potential, T in [6500, 7500], 2^20 float64-truth
-    samples (5 % T margin), weighted loss, GPU Adam 400 epochs, batch 1024, cosine lr
-    2e-3 -> 1e-6."""
+def train_cooling_surrogate(epochs: int = 800, n_samples: int = 1 << 20, seed: int = 0, log=None):
+    """The recipe for the 7350 -> 6770 K cooling runs: complex-field potential, T in
+    [6500, 7500], 2^20 float64-truth samples (5 % T margin), weighted loss (floored 1e-4,
+    barrier 1e-3), GPU Adam 800 epochs, batch 1024, cosine lr 2e-3 -> 1e-6."""
     spec = SurrogateSpec()
     rho, T = spec.sample(n_samples, seed=seed, T_margin=0.05)
     sur = Surrogate(spec, seed=seed)
