@@ -61,14 +61,16 @@ def surrogate_vs_table(surrogate, n: int = 64, steps: int = 4000, T0: float = 73
                        "elapsed_s": el, "ms_per_step": 1e3 * el / (steps * len(seeds)),
                        "table_rebuilds": rebuilds}
         out["steps_axis"] = list(range(report, steps + 1, report))
-    t, s = out["table"], out["surrogate"]
-
     def within(m1, s1, m2, s2):
         tol = np.sqrt(s1 ** 2 + s2 ** 2)
         return np.abs(m1 - m2) <= np.maximum(tol, 1e-12)
 
-    out["f_within_1sigma"] = within(t["f_mean"], t["f_sd"], s["f_mean"], s["f_sd"])
-    out["s_within_1sigma"] = within(t["s_mean"], t["s_sd"], s["s_mean"], s["s_sd"])
+    for ref in ("table", "table_exact"):
+        if ref in out and "surrogate" in out:
+            t, s = out[ref], out["surrogate"]
+            sfx = "" if ref == "table" else "_exact"
+            out["f_within_1sigma" + sfx] = within(t["f_mean"], t["f_sd"], s["f_mean"], s["f_sd"])
+            out["s_within_1sigma" + sfx] = within(t["s_mean"], t["s_sd"], s["s_mean"], s["s_sd"])
     return out
 
 
