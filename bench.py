@@ -233,11 +233,12 @@ def bench_ensemble(steps: int = 200, modes=("graph", "batched", "sequential")):
     ens.step(300, mode="batched")
     state = [(ens.download(i, "phi"), ens.download(i, "pi")) for i in range(len(shapes))]
     tune = autotune.tune_ensemble_joint(ens, steps=100, use_cache=False)
-    log(f"ensemble autotune best geometry {tune['best'][0]}, {tune['best'][1]} bytes/group: "
+    log(f"ensemble autotune best geometry {tune['best'][0]}, {tune['best'][1]} bytes/group, "
+        f"{tune['best'][2]} group(s) at once: "
         f"{tune['us_per_step']:.1f} us/step ({tune['table']})")
     sites = float(sum(int(np.prod(s)) for s in shapes))
     out = {"instances": len(shapes), "sites_per_step": sites, "autotune_best": tune["best"][0],
-           "group_bytes": tune["best"][1]}
+           "group_bytes": tune["best"][1], "groups_at_once": tune["best"][2]}
     for mode in modes:
         for i, (phi, pi) in enumerate(state):
             ens.upload(i, "phi", phi)

@@ -468,6 +468,10 @@ int tip_ensemble_set_geometry(tip_ensemble *ens, int kernel, int rows, int plane
  * the GPU's last-level cache (Infinity Cache) from one step to the next.  0 (default) = one
  * launch per step per class.  Results are identical either way. */
 int tip_ensemble_set_schedule(tip_ensemble *ens, int64_t group_bytes);
+/* Groups stepping at once in the batched and graph modes: 1 (default) or 2 (groups alternate
+ * between two streams, so two independent groups run concurrently and one group's launch
+ * tails share the GPU with the other's blocks).  Results are identical either way. */
+int tip_ensemble_set_concurrency(tip_ensemble *ens, int groups_at_once);
 int tip_ensemble_step(tip_ensemble *ens, int64_t nsteps, int mode, float *elapsed_ms);
 int tip_ensemble_vacuum_counts(tip_ensemble *ens, double esc, unsigned long long *host_counts);
 int tip_ensemble_step_count(tip_ensemble *ens, uint64_t *step);

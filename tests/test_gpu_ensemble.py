@@ -120,21 +120,24 @@ def test_row_mappings_several_rows_per_thread(ty, group_bytes):
 
 
 @pytest.mark.parametrize("group_bytes", [1, 40000, 1 << 30])
-def test_grouped_schedule_identical(group_bytes):
-    """Instances stepped group by group (each group through all steps of a call) give the same
-    fields as one launch per step over all instances."""
-    ens_a, _, _, _ = _setup()
-    ens_a.step(5, mode="batched")
-    ref = _state(ens_a)
+@pytest.mark.parametrize("concurrency", [1, 2])
+def test_grouped_schedule_identical(group_bytes, concurrency):
+    """Instances stepped group by group (each group through all steps of a call; with
+    concurrency 2, two groups at once on two streams) give the same fields as one launch per
+    step over all instances."""
     for mode in ("batched", "graph"):
         ens_b, _, _, _ = _setup()
         ens_b.set_schedule(group_bytes)
+        ens_b.set_concurrency(concurrency)
         ens_b.step(2, mode=mode)
         ens_b.step(3, mode=mode)
-        assert _same(_state(ens_b), ref), (group_bytes, mode)
-        assert ens_b.step_count == 5
+        ens_b.step(19, mode=mode)  # a full 16-step graph chunk and a remainder
+        ens_a2, _, _, _ = _setup()
+        ens_a2.step(24, mode="batched")
+        assert _same(_state(ens_b), _state(ens_a2)), (group_bytes, concurrency, mode)
+        assert ens_b.step_count == 24
         ens_b.close()
-    ens_a.close()
+        ens_a2.close()
 
 
 def test_graph_replays_each_checked():
@@ -183,9 +186,9 @@ def test_joint_tuner_picks_a_pair_and_keeps_results(tmp_path, monkeypatch):
     ens, _, _, _ = _setup()
     rec = autotune.tune_ensemble_joint(ens, geometries=geoms, group_sizes=(0, 1, 40000),
                                        top=2, steps=2)
-    g, gb = rec["best"]
-    assert tuple(g) in geoms and gb in (0, 1, 40000)
-    assert len(rec["table"]) == 2 * 3 and len(rec["geometries"]) == 3
+    g, gb, conc = rec["best"]
+    assert tuple(g) in geoms and gb in (0, 1, 40000) and conc in (1, 2)
+    assert len(rec["table"]) == 2 * 3 * 2 and len(rec["geometries"]) == 3
     rec2 = autotune.tune_ensemble_joint(ens, geometries=[(1, 3, 0, 0, 1)],
                                         group_sizes=(0, 1, 40000), top=2, steps=2)
     assert rec2["best"] == rec["best"]  # served from the cache
@@ -195,6 +198,7 @@ def test_joint_tuner_picks_a_pair_and_keeps_results(tmp_path, monkeypatch):
     ens_a.step(4, mode="batched")
     ens_b, _, _, _ = _setup(geometry=tuple(g))
     ens_b.set_schedule(gb)
+    ens_b.set_concurrency(conc)
     ens_b.step(4, mode="batched")
     assert _same(_state(ens_b), _state(ens_a))
     ens_a.close()

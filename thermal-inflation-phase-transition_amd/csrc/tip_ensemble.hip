@@ -137,6 +137,9 @@ struct tip_ensemble {
     uint64_t step = 0;
     int parity = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;            // second stream of the concurrent schedule
+    hipEvent_t fork = nullptr, join = nullptr;  // ordering between the two streams
+    int concurrency = 1;                        // groups stepping at once (1 or 2)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* dev_vac = nullptr;
     // geometry: kind (TIP_KERNEL_*), by = rows (flat: 64-site rows per block; row tiles: y rows
@@ -177,8 +180,12 @@ int release(tip_ensemble* E)
         if (p) (void)hipFree(p);
     for (auto p : E->table_ab)
         if (p) (void)hipFree(p);
+    if (E->stream2) (void)hipStreamSynchronize(E->stream2);
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->fork) (void)hipEventDestroy(E->fork);
+    if (E->join) (void)hipEventDestroy(E->join);
+    if (E->stream2) (void)hipStreamDestroy(E->stream2);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
     return TIP_OK;
@@ -328,8 +335,9 @@ int build_items(tip_ensemble* E)
     return TIP_OK;
 }
 
-// One step of the items of group g.
-int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g)
+// One step of the items of group g on stream st.
+int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g,
+                   hipStream_t st)
 {
     const EnsItem* items = E->dev_items + g.first;
     const int nit = g.last - g.first;
@@ -344,7 +352,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g)
                     "ensemble: bad block size %d", g.threads);
 #define TIP_LAUNCH_ENSR_C(TYV, ODD, DENSE)                                                     \
     hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD, false, DENSE>), dim3(nb),              \
-                       dim3(64, g.threads / 64), lds, E->stream, E->dev_inst, items, nit, E->cx, \
+                       dim3(64, g.threads / 64), lds, st, E->dev_inst, items, nit, E->cx, \
                        parity, E->dev_step, step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENSR(TYV, ODD)                 \
     do {                                          \
@@ -371,7 +379,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g)
         return TIP_OK;
     }
 #define TIP_LAUNCH_ENS(BYV, ODD)                                                                \
-    hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, E->stream,     \
+    hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, st,            \
                        E->dev_inst, items, nit, E->cx, parity, E->dev_step,                    \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENS_BY(ODD)                         \
@@ -392,14 +400,29 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g)
     return TIP_OK;
 }
 
-// All launches of nsteps steps starting at buffer parity par0 (no counter bump).
+// All launches of nsteps steps starting at buffer parity par0 (no counter bump).  With
+// concurrency 2 the groups alternate between the ensemble's stream and a second stream (forked
+// from the first and joined back at the end), so two independent groups step at once and
+// the last blocks of one group's launch share the GPU with the other group's blocks; each
+// group's steps stay in order on its own stream.
 int issue_steps(tip_ensemble* E, int64_t nsteps, int par0)
 {
-    for (const auto& g : E->groups)
+    const bool two = E->concurrency == 2 && E->groups.size() > 1;
+    if (two) {
+        TIP_HIP(hipEventRecord(E->fork, E->stream));
+        TIP_HIP(hipStreamWaitEvent(E->stream2, E->fork, 0));
+    }
+    for (size_t gi = 0; gi < E->groups.size(); ++gi) {
+        const hipStream_t st = (two && (gi & 1)) ? E->stream2 : E->stream;
         for (int64_t s = 0; s < nsteps; ++s) {
-            int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, g);
+            int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, E->groups[gi], st);
             if (rc) return rc;
         }
+    }
+    if (two) {
+        TIP_HIP(hipEventRecord(E->join, E->stream2));
+        TIP_HIP(hipStreamWaitEvent(E->stream, E->join, 0));
+    }
     return TIP_OK;
 }
 
@@ -478,6 +501,9 @@ extern "C" int tip_ensemble_create(tip_ensemble** out, int n, const int64_t* dim
     E->has_table.assign(n, 0);
     E->has_params.assign(n, 0);
     hipError_t e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&E->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&E->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&E->ev0);
     if (e == hipSuccess) e = hipEventCreate(&E->ev1);
     if (e == hipSuccess) e = hipMalloc(&E->pool, 4 * E->total * sizeof(float));
@@ -607,6 +633,19 @@ extern "C" int tip_ensemble_set_schedule(tip_ensemble* E, int64_t group_bytes)
     TIP_REQUIRE(group_bytes >= 0, "tip_ensemble_set_schedule: group_bytes < 0");
     E->group_bytes = group_bytes;
     E->items_dirty = true;
+    return TIP_OK;
+}
+
+extern "C" int tip_ensemble_set_concurrency(tip_ensemble* E, int groups_at_once)
+{
+    TIP_REQUIRE(E, "tip_ensemble_set_concurrency: NULL");
+    TIP_REQUIRE(groups_at_once == 1 || groups_at_once == 2,
+                "tip_ensemble_set_concurrency: 1 or 2 groups at once (got %d)", groups_at_once);
+    if (groups_at_once != E->concurrency) {
+        TIP_HIP(hipStreamSynchronize(E->stream));
+        drop_graphs(E);  // captured graphs hold the previous stream layout
+        E->concurrency = groups_at_once;
+    }
     return TIP_OK;
 }
 

@@ -157,6 +157,7 @@ SIGNATURES = {
     "tip_ensemble_set_table": (_int, [_vp, _int, _vp, _i64, _d, _d, _int]),
     "tip_ensemble_set_params": (_int, [_vp, _int, C.POINTER(StepParams)]),
     "tip_ensemble_set_geometry": (_int, [_vp, _int, _int, _int, _int, _int]),
+    "tip_ensemble_set_concurrency": (_int, [_vp, _int]),
     "tip_ensemble_set_schedule": (_int, [_vp, _i64]),
     "tip_ensemble_step": (_int, [_vp, _i64, _int, C.POINTER(C.c_float)]),
     "tip_ensemble_vacuum_counts": (_int, [_vp, _d, _vp]),

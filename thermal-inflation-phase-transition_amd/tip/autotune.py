@@ -139,40 +139,48 @@ def tune_ensemble_joint(ens, geometries: Optional[Iterable] = None,
                         group_sizes: Iterable[int] = (0, 96 << 20, 160 << 20, 224 << 20,
                                                       320 << 20),
                         probe_group: int = 224 << 20, top: int = 3, steps: int = 100,
-                        mode: str = "batched", use_cache: bool = True) -> dict:
+                        mode: str = "batched", use_cache: bool = True,
+                        concurrency: Iterable[int] = (1, 2)) -> dict:
     """Geometry and step schedule together, in two stages: every geometry under the
-    `probe_group` schedule, then the `top` fastest geometries with every group size.  The
-    geometry and the group size interact (a small group under-fills the GPU when each block
-    marches many planes), so tuning one with the other fixed can miss the best pair.  Tune on
-    a field state like the one to be stepped (the kernels' time depends on the state)."""
+    `probe_group` schedule, then the `top` fastest geometries with every group size and every
+    number of groups stepping at once (`concurrency`).  The geometry and the group size
+    interact (a small group under-fills the GPU when each block marches many planes, unless
+    a second group runs beside it), so tuning one with the others fixed can miss the best
+    combination.  Tune on a field state like the one to be stepped (the kernels' time depends
+    on the state).  best = [geometry, group bytes, groups at once]."""
     gsz = [int(g) for g in group_sizes]
+    conc = [int(c) for c in concurrency]
     key = (f"ensemble_joint:{sorted(set(ens.shapes))}:{len(ens.shapes)}:{mode}:{steps}:"
-           f"{sorted(gsz)}")
+           f"{sorted(gsz)}:{sorted(conc)}")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
             ens.set_geometry(*hit["best"][0])
             ens.set_schedule(hit["best"][1])
+            ens.set_concurrency(hit["best"][2])
             return hit
 
-    def timed(geom, gb):
+    def timed(geom, gb, c):
         ens.set_geometry(*geom)
         ens.set_schedule(gb)
+        ens.set_concurrency(c)
         ens.step(steps, mode=mode, timed=True)  # warm (and capture, in graph mode)
         return ens.step(steps, mode=mode, timed=True) / steps
 
-    stage1 = sorted((timed(tuple(g), int(probe_group)), tuple(g))
+    stage1 = sorted((timed(tuple(g), int(probe_group), 1), tuple(g))
                     for g in (geometries or ensemble_candidates()))
     results = []
     for _, g in stage1[:max(1, int(top))]:
         for gb in gsz:
-            results.append((timed(g, gb), g, gb))
+            for c in conc:
+                results.append((timed(g, gb, c), g, gb, c))
     results.sort()
-    _, g_best, gb_best = results[0]
+    _, g_best, gb_best, c_best = results[0]
     ens.set_geometry(*g_best)
     ens.set_schedule(gb_best)
-    rec = {"best": [list(g_best), gb_best], "us_per_step": results[0][0] * 1e3,
-           "table": [[list(g), gb, t * 1e3] for t, g, gb in results],
+    ens.set_concurrency(c_best)
+    rec = {"best": [list(g_best), gb_best, c_best], "us_per_step": results[0][0] * 1e3,
+           "table": [[list(g), gb, c, t * 1e3] for t, g, gb, c in results],
            "geometries": [[list(g), t * 1e3] for t, g in stage1]}
     _store_cache(key, rec)
     return rec

@@ -63,6 +63,10 @@ class Ensemble:
         all steps of a call before the next (0 = one group per class)."""
         _lib.call("tip_ensemble_set_schedule", self._h, int(group_bytes))
 
+    def set_concurrency(self, groups_at_once: int = 1) -> None:
+        """1 or 2 groups of the schedule stepping at once (two streams); same results."""
+        _lib.call("tip_ensemble_set_concurrency", self._h, int(groups_at_once))
+
     def step(self, nsteps: int, mode: str = "graph", timed: bool = False):
         ms = C.c_float(0.0)
         _lib.call("tip_ensemble_step", self._h, int(nsteps), MODES[mode],
