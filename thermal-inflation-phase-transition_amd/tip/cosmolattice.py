@@ -3,8 +3,8 @@ runs of the reference's CosmoLattice port.
 
   write_raw_snapshot / read_raw_snapshot   FLPI / FLP2 files of
       cosmolattice_ext/measurements/field_snapshot.hpp:38-56 (packed header, then float32
-      N^3 payloads phi [phi2] pi [pi2] in program units, value = GeV / fStar), read as
-      tools/export_cl_snapshots.py:43-122 reads them
+      N^3 payloads phi [phi2] pi [pi2] in program units, value = GeV / fStar); the reader
+      returns the dict of tools/export_cl_snapshots.py:43-122
   append_manifest / read_manifest          field_states/manifest.csv (field_snapshot.hpp:296-321)
   export_npz                               the numba-driver npz the exporter writes
                                            (export_cl_snapshots.py:236-275); winding and
@@ -72,49 +72,55 @@ def write_raw_snapshot(path: str, fields: Dict[str, np.ndarray], *, step: int, t
             f.write(np.ascontiguousarray(x, dtype="<f4").tobytes())
 
 
+# Payload order of each magic, as FieldSnapshotWriter lays the fields out after the packed
+# header (field_snapshot.hpp:38-56): the header format and the field names in file order.
+_LAYOUT = {
+    MAGIC_PHI: (HEADER_FMT, ("phi",)),
+    MAGIC_PI: (HEADER_FMT, ("phi", "pi")),
+    MAGIC_PI2: (HEADER2_FMT, ("phi1", "phi2", "pi1", "pi2")),
+}
+
+
 def read_raw_snapshot(path: str, units: str = "GeV", derived: bool = True) -> dict:
-    """Read an FLPH / FLPI / FLP2 snapshot into the exporter's dict (fields in GeV as float64,
-    program-unit float32 with units="program").  For two components and derived=True, rho,
-    theta and the winding buffer are computed on the GPU from the float64 fields."""
-    with open(path, "rb") as f:
-        head = f.read(4)
-        if len(head) < 4:
-            raise ValueError(f"truncated file: {path}")
-        magic = struct.unpack("<I", head)[0]
-        f.seek(0)
-        fmt = HEADER2_FMT if magic == MAGIC_PI2 else HEADER_FMT
-        buf = f.read(struct.calcsize(fmt))
-        if len(buf) != struct.calcsize(fmt):
-            raise ValueError(f"truncated header: {path}")
-        vals = struct.unpack(fmt, buf)
-        magic, n, step, t, T, a, H, fStar = vals[:8]
-        if magic not in (MAGIC_PHI, MAGIC_PI, MAGIC_PI2):
-            raise ValueError(f"bad magic {magic:#x} in {path}")
-        n_comp = int(vals[8]) if magic == MAGIC_PI2 else 1
-        n3 = n * n * n
+    """Read an FLPH / FLPI / FLP2 snapshot into a dict with the exporter's keys: the fields in
+    GeV as float64 (program-unit float32 with units="program") and the header scalars.  For
+    two components and derived=True, rho, theta and the winding buffer are added; the
+    winding is computed on the GPU from the float64 fields.
 
-        def field():
-            payload = f.read(4 * n3)
-            if len(payload) != 4 * n3:
-                raise ValueError(f"truncated field payload in {path}")
-            x = np.frombuffer(payload, dtype="<f4").reshape((n, n, n))
-            return x.astype(np.float64) * fStar if units == "GeV" else x.astype(np.float32)
-
-        out = dict(step=int(step), time=float(t), temperature=float(T), a=float(a), H=float(H),
-                   fStar=float(fStar), N=int(n), n_scalars=2 if n_comp >= 2 else 1)
-        if n_comp >= 2:
-            out["phi1"], out["phi2"] = field(), field()
-            out["pi1"], out["pi2"] = field(), field()
-            if derived and units == "GeV":
-                from .kernels_complex import compute_winding_number
-                p1, p2 = out["phi1"], out["phi2"]
-                out["rho"] = np.sqrt(p1 ** 2 + p2 ** 2)
-                out["theta"] = np.arctan2(p2, p1)
-                out["winding"] = compute_winding_number(p1, p2)[0]
-        else:
-            out["phi"] = field()
-            if magic == MAGIC_PI:
-                out["pi"] = field()
+    The whole file is read once; its length must equal header + fields x 4 N^3 bytes exactly,
+    so a short or over-long file is rejected before any field is decoded."""
+    if units not in ("GeV", "program"):
+        raise ValueError(f"unknown units {units!r}")
+    blob = open(path, "rb").read()
+    if len(blob) < 4:
+        raise ValueError(f"{path}: {len(blob)} bytes, too short for a snapshot magic")
+    (magic,) = struct.unpack_from("<I", blob, 0)
+    if magic not in _LAYOUT:
+        raise ValueError(f"{path}: magic {magic:#010x} is not FLPH, FLPI or FLP2")
+    fmt, names = _LAYOUT[magic]
+    hsize = struct.calcsize(fmt)
+    if len(blob) < hsize:
+        raise ValueError(f"{path}: {len(blob)} bytes, header needs {hsize}")
+    head = struct.unpack_from(fmt, blob, 0)
+    n, step, t, T, a, H, fStar = head[1:8]
+    n3 = int(n) ** 3
+    want = hsize + 4 * n3 * len(names)
+    if len(blob) != want:
+        raise ValueError(f"{path}: {len(blob)} bytes, N={n} with {len(names)} fields needs "
+                         f"{want}")
+    data = np.frombuffer(blob, dtype="<f4", count=n3 * len(names), offset=hsize)
+    data = data.reshape((len(names), n, n, n))
+    out = dict(step=int(step), time=float(t), temperature=float(T), a=float(a), H=float(H),
+               fStar=float(fStar), N=int(n), n_scalars=2 if magic == MAGIC_PI2 else 1)
+    for i, name in enumerate(names):
+        out[name] = (data[i].astype(np.float64) * fStar if units == "GeV"
+                     else data[i].astype(np.float32))
+    if magic == MAGIC_PI2 and derived and units == "GeV":
+        from .kernels_complex import compute_winding_number
+        p1, p2 = out["phi1"], out["phi2"]
+        out["rho"] = np.sqrt(p1 ** 2 + p2 ** 2)
+        out["theta"] = np.arctan2(p2, p1)
+        out["winding"] = compute_winding_number(p1, p2)[0]
     return out
 
 
