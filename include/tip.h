@@ -9,6 +9,10 @@
  *    Fields are C-contiguous (nx, ny, nz) arrays, z fastest, exactly the reference's numpy
  *    layout.  `stream` is a hipStream_t (NULL = the default stream); calls are asynchronous
  *    with respect to the host unless stated otherwise.
+ *  - Library scratch (the float32 drop-ins' prepared segment table, reductions, labelling)
+ *    is one buffer per device and purpose.  A call on another stream than the buffer's last
+ *    user waits on the device for that use to end, so calls on several streams are safe;
+ *    calls from several host threads at once must be serialised by the caller.
  *  - Scalars follow the reference's meaning and units (rescaled lattice units: dx = mu*dx_phys,
  *    dt = mu*dt_phys, eta_eff, mu, inv_a2 = 1/a^2).
  *  - Noise: a non-NULL `noise` array is added as the reference adds its pre-generated field

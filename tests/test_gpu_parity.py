@@ -231,6 +231,59 @@ def test_f32_matches_oracle_f32mixed_one_step():
     assert rel(g, a) < 1e-6 and rel(h, b) < 1e-6
 
 
+def test_f32_dropin_two_streams_two_tables():
+    """The float32 drop-in prepares its segment table in library scratch shared by every
+    call on the device.  Calls on two streams with different tables, issued back to back,
+    must each step with their own table: bit-identical to the same calls made one after the
+    other on one stream."""
+    from tip import _lib
+    import ctypes as C
+    tip, K, _, DA, _ = _tip()
+    shape = (128, 128, 128)
+    nx, ny, nz = shape
+    prm = real_params()
+    t1, tmin1, dinv1 = real_table()
+    t2 = 1.5 * t1
+    tabs = [(DA.from_numpy(t1), tmin1, dinv1), (DA.from_numpy(t2), tmin1, dinv1)]
+    rng = np.random.default_rng(11)
+    init = [((rng.normal(size=shape) * 3e3).astype(np.float32),
+             (rng.normal(size=shape) * 1e3).astype(np.float32)) for _ in range(2)]
+
+    def fields():
+        return [(DA.from_numpy(a), DA.from_numpy(b), DA.empty(shape, np.float32),
+                 DA.empty(shape, np.float32)) for a, b in init]
+
+    def step(f, tab, seed, stream):
+        phi, pi, pt, qt = f
+        _lib.call("tip_rk2_step_table_f32", phi.ptr, pi.ptr, prm["dt"], prm["dx"],
+                  prm["eta_eff"], prm["mu"], None, pt.ptr, qt.ptr, 1.0, tab[0].ptr, tab[1],
+                  tab[2], len(t1), 1, nx, ny, nz, seed, prm["noise_scale"], None, 1e4, stream)
+
+    streams = []
+    for _ in range(2):
+        h = C.c_void_p()
+        _lib.call("tip_stream_create", C.byref(h))
+        streams.append(h.value)
+    try:
+        ref = fields()
+        for k in range(3):
+            for i in range(2):
+                step(ref[i], tabs[i], 100 + k, streams[0])
+        _lib.call("tip_stream_synchronize", streams[0])
+        got = fields()
+        for k in range(3):
+            for i in range(2):
+                step(got[i], tabs[i], 100 + k, streams[i])
+        for s_ in streams:
+            _lib.call("tip_stream_synchronize", s_)
+        for i in range(2):
+            for a, b in zip(got[i][:2], ref[i][:2]):
+                assert np.array_equal(a.numpy(), b.numpy())
+    finally:
+        for s_ in streams:
+            _lib.call("tip_stream_destroy", s_)
+
+
 def test_fused_vacuum_count_matches_standalone():
     tip, K, _, DA, Counter = _tip()
     rng = np.random.default_rng(11)

@@ -40,25 +40,47 @@ int check_dims(int64_t nx, int64_t ny, int64_t nz)
 }
 
 // Scratch buffers per (device, slot).  Growing a buffer synchronises the device first so
-// no kernel still reads the old one.
+// no kernel still reads the old one.  Each slot also keeps an event recorded on the stream of
+// its last use (scratch_release); a call on another stream waits for that event before it
+// writes the slot, so entry points on different streams never overwrite a buffer a kernel is
+// still reading.  Streams under graph capture neither wait nor record.
 namespace {
 std::mutex g_scratch_mu;
 struct Scratch {
     void* p = nullptr;
     size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool recorded = false;
 };
 std::vector<std::vector<Scratch>> g_scratch;
-}  // namespace
 
-void* scratch_get(size_t bytes, int slot)
+bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+Scratch* scratch_slot(int slot)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
     if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
     auto& v = g_scratch[dev];
     if ((int)v.size() <= slot) v.resize(slot + 1);
-    Scratch& s = v[slot];
+    return &v[slot];
+}
+}  // namespace
+
+void* scratch_get(size_t bytes, int slot, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    Scratch* sp = scratch_slot(slot);
+    if (!sp) return nullptr;
+    Scratch& s = *sp;
+    if (s.recorded && s.last != st && !capturing(st) &&
+        hipStreamWaitEvent(st, s.ev, 0) != hipSuccess)
+        return nullptr;
     if (s.bytes >= bytes && s.p) return s.p;
     if (s.p) {
         if (hipDeviceSynchronize() != hipSuccess) return nullptr;
@@ -73,6 +95,22 @@ void* scratch_get(size_t bytes, int slot)
     }
     s.bytes = want;
     return s.p;
+}
+
+void scratch_release(int slot, hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    Scratch* sp = scratch_slot(slot);
+    if (!sp || capturing(st)) return;
+    Scratch& s = *sp;
+    if (!s.ev && hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) {
+        s.ev = nullptr;
+        return;
+    }
+    if (hipEventRecord(s.ev, st) == hipSuccess) {
+        s.last = st;
+        s.recorded = true;
+    }
 }
 
 }  // namespace tip
@@ -741,9 +779,10 @@ static int minmax(const R* a, const R* b, int64_t n, double* out, void* stream)
     int64_t nb64 = (nv + 256 * kVacU - 1) / (256 * kVacU);
     nb64 = nb64 < 1 ? 1 : (nb64 > 1024 ? 1024 : nb64);
     const unsigned nb = (unsigned)nb64;
-    double* part = reinterpret_cast<double*>(scratch_get(sizeof(double) * 2 * nb, 1));
-    TIP_REQUIRE(part, "tip_minmax: scratch allocation failed");
     hipStream_t st = as_stream(stream);
+    ScratchLease lease{1, st};
+    double* part = reinterpret_cast<double*>(scratch_get(sizeof(double) * 2 * nb, 1, st));
+    TIP_REQUIRE(part, "tip_minmax: scratch allocation failed");
     if (vec)
         hipLaunchKernelGGL((k_minmax_part<R, MODE, L>), dim3(nb), dim3(256), 0, st, a, b, n,
                            part);

@@ -45,8 +45,19 @@ constexpr int64_t kMaxSites = int64_t(1) << 31;
 int check_dims(int64_t nx, int64_t ny, int64_t nz);
 
 // Library-owned device scratch, grown on demand (used by drop-in entry points that need a
-// converted table or a counter).  One per device.
-void* scratch_get(size_t bytes, int slot);
+// converted table or a counter).  One buffer per (device, slot).  scratch_get makes stream st
+// wait for the slot's last use on another stream; scratch_release records the end of this
+// use on st (ScratchLease does it when the entry point returns, after its launches).
+// Calls from several host threads at once that use one slot must be serialised by the
+// caller: the slot is one buffer.
+void* scratch_get(size_t bytes, int slot, hipStream_t st);
+void scratch_release(int slot, hipStream_t st);
+
+struct ScratchLease {
+    int slot;
+    hipStream_t st;
+    ~ScratchLease() { scratch_release(slot, st); }
+};
 
 // ---------------------------------------------------------------------------------------
 // Counter-based hash RNG of the reference (simulation/latticeSimeRescale_numba.py:2322-2336).

@@ -401,8 +401,10 @@ int label(const R* v, double thr, int64_t nx, int64_t ny, int64_t nz, int period
     hipStream_t st = as_stream(stream);
     const int64_t n = nx * ny * nz;
     const int64_t ntiles = (n + kTile - 1) / kTile;
-    int32_t* ids = reinterpret_cast<int32_t*>(scratch_get((size_t)n * sizeof(int32_t), 3));
-    char* aux = reinterpret_cast<char*>(scratch_get((size_t)ntiles * sizeof(int32_t) + 64, 4));
+    ScratchLease lease_ids{3, st}, lease_aux{4, st};
+    int32_t* ids = reinterpret_cast<int32_t*>(scratch_get((size_t)n * sizeof(int32_t), 3, st));
+    char* aux =
+        reinterpret_cast<char*>(scratch_get((size_t)ntiles * sizeof(int32_t) + 64, 4, st));
     TIP_REQUIRE(ids && aux, "tip_label: scratch allocation failed");
     int64_t* total = reinterpret_cast<int64_t*>(aux);
     int32_t* tiles = reinterpret_cast<int32_t*>(aux + 64);

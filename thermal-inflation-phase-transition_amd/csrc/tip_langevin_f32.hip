@@ -124,7 +124,8 @@ int langevin_complex_f32(int op, const float* phi1, const float* phi2, const flo
 namespace {
 
 // The float32 kernels read the (A, B) segment form of the float64 device table, prepared into
-// the per-device scratch (slot 0, as the float32 RK2 drop-in).
+// the per-device scratch (slot 0, as the float32 RK2 drop-in).  The caller holds a
+// ScratchLease on slot 0 across its launches.
 int prepared_table(const char* who, int64_t nx, int64_t ny, int64_t nz, const double* vp_table,
                    int64_t npts, double tmin, double dinv, int odd, double eta, hipStream_t st,
                    SegTable& out)
@@ -134,7 +135,8 @@ int prepared_table(const char* who, int64_t nx, int64_t ny, int64_t nz, const do
     TIP_REQUIRE(vp_table, "%s: NULL table", who);
     TIP_REQUIRE(npts >= 2 && npts < (int64_t(1) << 30), "%s: vp_npts out of range", who);
     TIP_REQUIRE(eta > 0.0, "%s: eta / gamma must be positive", who);
-    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(npts - 1) * sizeof(float2), 0));
+    float2* ab =
+        reinterpret_cast<float2*>(scratch_get((size_t)(npts - 1) * sizeof(float2), 0, st));
     TIP_REQUIRE(ab != nullptr, "%s: scratch allocation failed", who);
     rc = prepare_table_f32(vp_table, npts, tmin, dinv, ab, st);
     if (rc) return rc;
@@ -167,6 +169,7 @@ extern "C" int tip_overdamped_f32(const float* phi, float* phi_out, double dt, d
     hipStream_t st = as_stream(stream);
     TIP_REQUIRE(phi && phi_out && phi != phi_out, "overdamped f32: NULL or aliased field");
     SegTable tab;
+    ScratchLease lease{0, st};
     int rc = prepared_table("overdamped f32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv,
                             vp_odd, eta_eff, st, tab);
     if (rc) return rc;
@@ -187,6 +190,7 @@ extern "C" int tip_baoab_f32(int stage, const float* phi, const float* pi, float
     int rc = check_stage("baoab f32", stage, phi, pi, phi_out, pi_out);
     if (rc) return rc;
     SegTable tab;
+    ScratchLease lease{0, st};
     rc = prepared_table("baoab f32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv, vp_odd,
                         gamma, st, tab);
     if (rc) return rc;
@@ -208,6 +212,7 @@ extern "C" int tip_overdamped_c32(const float* phi1, const float* phi2, float* p
     TIP_REQUIRE(phi1 && phi2 && phi1_out && phi2_out && phi1 != phi1_out && phi2 != phi2_out,
                 "overdamped c32: NULL or aliased field");
     SegTable tab;
+    ScratchLease lease{0, st};
     int rc = prepared_table("overdamped c32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv,
                             0, eta_eff, st, tab);
     if (rc) return rc;
@@ -233,6 +238,7 @@ extern "C" int tip_baoab_c32(int stage, const float* phi1, const float* phi2, co
     rc = check_stage("baoab c32", stage, phi2, pi2, phi2_out, pi2_out);
     if (rc) return rc;
     SegTable tab;
+    ScratchLease lease{0, st};
     rc = prepared_table("baoab c32", nx, ny, nz, vp_table, vp_npts, vp_tmin, vp_dinv, 0, gamma,
                         st, tab);
     if (rc) return rc;

@@ -262,7 +262,9 @@ int train_pass(const float* P, const float* X, const float* Y, const float* Wt, 
     TIP_REQUIRE(P && X && Y, "mlp: NULL parameters or data");
     TIP_REQUIRE(n >= 1 && n < (int64_t(1) << 30), "mlp: batch size out of range");
     const int nb = (int)((n + SB - 1) / SB);
-    float* part = reinterpret_cast<float*>(scratch_get(((size_t)nb * NP + nb) * sizeof(float), 2));
+    ScratchLease lease{2, st};
+    float* part =
+        reinterpret_cast<float*>(scratch_get(((size_t)nb * NP + nb) * sizeof(float), 2, st));
     TIP_REQUIRE(part, "mlp: scratch allocation failed");
     float* lpart = part + (size_t)nb * NP;
     const float inv_n = (float)(1.0 / (double)n);

@@ -369,7 +369,9 @@ extern "C" int tip_rk2_step_table_f32(float* phi, float* pi, double dt, double d
     TIP_REQUIRE(vp_table, "rk2_step_table_f32: NULL table");
     int rc = check_npts(vp_npts);
     if (rc) return rc;
-    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
+    ScratchLease lease{0, st};
+    float2* ab =
+        reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0, st));
     TIP_REQUIRE(ab != nullptr, "rk2_step_table_f32: scratch allocation failed");
     rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
     if (rc) return rc;
@@ -420,7 +422,9 @@ extern "C" int tip_rk2_step_table_c32(
     TIP_REQUIRE(vp_table, "rk2 c32: NULL table");
     int rc = check_npts(vp_npts);
     if (rc) return rc;
-    float2* ab = reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0));
+    ScratchLease lease{0, st};
+    float2* ab =
+        reinterpret_cast<float2*>(scratch_get((size_t)(vp_npts - 1) * sizeof(float2), 0, st));
     TIP_REQUIRE(ab != nullptr, "rk2 c32: scratch allocation failed");
     rc = prepare_table_f32(vp_table, vp_npts, vp_tmin, vp_dinv, ab, st);
     if (rc) return rc;
