@@ -782,11 +782,17 @@ __device__ __forceinline__ void gauss_pair_f32_l2(uint64_t seed, uint32_t word_m
 
 // V'(rho) of the complex field's radial potential: the float32 segment table ...
 // fetch() issues the gather, eval() finishes V' from what it returned, so a caller can issue
-// the gathers of several sites before it needs any of them.
+// the gathers of several sites before it needs any of them.  A Radial with kBatch instead
+// evaluates a whole plane's radii (both stages, every row of the wave) in one wave-collective
+// batch() call after the next plane's loads are issued; kLdsFloats of LDS past the staging
+// area are its own, filled once per block by block_init().
 struct TableRadial {
     TableF32<false> t;
     using Pre = float2;
     static constexpr bool kEarly = true;  // issue the gathers of a plane before its loads
+    static constexpr bool kBatch = false;
+    static constexpr int kLdsFloats = 0;
+    __device__ __forceinline__ void block_init(float*) {}
     __device__ __forceinline__ Pre fetch(float rho) const
     {
         const __amdgpu_buffer_rsrc_t T = make_rsrc(t.ab, (uint32_t)(t.last + 1) * 8u);
@@ -814,6 +820,9 @@ struct MlpGridRadial {
     // three registers per pending gather (and its log and sqrt) cost a wave per SIMD here:
     // the gathers are issued where they are used
     static constexpr bool kEarly = false;
+    static constexpr bool kBatch = false;
+    static constexpr int kLdsFloats = 0;
+    __device__ __forceinline__ void block_init(float*) {}
     __device__ __forceinline__ Pre fetch(float rho) const
     {
         const __amdgpu_buffer_rsrc_t Y = make_rsrc(seg, (uint32_t)(last + 1) * 8u);
@@ -984,6 +993,16 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 hn[NF + f] = buf_load_s(at(F[f], xa), vo, sop);
             }
         }
+        if constexpr (Radial::kBatch) {
+            // every lane of the wave takes part (idle lanes carry radii of row offset 0)
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                rho1[r] = complex_rho(vc[0][r], vc[1][r]);
+                rho2[r] = complex_rho(fmaf(c.half_dt, vc[2][r], vc[0][r]),
+                                      fmaf(c.half_dt, vc[3][r], vc[1][r]));
+            }
+            rad.template batch<TY>(rho1, rho2, P1, P2);
+        }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
@@ -1009,7 +1028,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 gauss_pair_f32_l2((col0K + (uint64_t)r * rowK + xK) ^ noise_seed, c.word_mask, g1,
                                   g2);
             float d1, d2;
-            if constexpr (Radial::kEarly)
+            if constexpr (Radial::kEarly || Radial::kBatch)
                 complex_force_finish(rad, zn, a, bb, rho1[r], P1[r], d1, d2);
             else
                 complex_force_f32(rad, zn, a, bb, d1, d2);
@@ -1019,7 +1038,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             const float t2a = fmaf(c.half_dt, fmaf(-6.0f, u, sum[2]), t1a);
             const float t2b = fmaf(c.half_dt, fmaf(-6.0f, v, sum[3]), t1b);
             float e1, e2;
-            if constexpr (Radial::kEarly)
+            if constexpr (Radial::kEarly || Radial::kBatch)
                 complex_force_finish(rad, zn, fa, fb, rho2[r], P2[r], e1, e2);
             else
                 complex_force_f32(rad, zn, fa, fb, e1, e2);
@@ -1081,12 +1100,15 @@ __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     unsigned long long* __restrict__ vac, float esc2)
 {
     extern __shared__ float lds_crows[];
+    Radial r = rad;
+    if constexpr (Radial::kLdsFloats > 0)
+        r.block_init(lds_crows + (size_t)blockDim.y * 64 * (8 * TY + 1));
     int tp, ch;
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
     unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT>(
-        phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, rad, zn, nx, ny, nz, W, tp, x0,
+        phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, r, zn, nx, ny, nz, W, tp, x0,
         x1, noise_seed, esc2, lds_crows);
     if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);
 }

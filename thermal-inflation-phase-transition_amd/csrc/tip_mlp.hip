@@ -100,16 +100,132 @@ __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, float x0,
     return y;
 }
 
-// V'(rho) at a fixed temperature -- the drop-in for the table functor in the step kernel.
-struct MlpForce {
+// V'(rho) at a fixed temperature inside the complex row-tile step, the network's 32 x 32 layer
+// on the matrix cores.  A wave evaluates its sites 16 at a time: for a block of 16 sites
+// (lanes 16 nb .. 16 nb + 15 hold their radii) lane l = 16 g + p computes the first layer for
+// site p and neurons 8 g + j (j = 0..7) on the VALU; those eight tanh values are its B operand
+// of v_mfma_f32_16x16x32_f16 (B[k = 8 g + j][n = p]), the A operand holds the weights
+// (A[m = p][k = 8 g + j] = W2[16 m2 + p][8 g + j], two 16-row halves m2), the accumulators
+// start at b2, and lane l receives z2 of neurons 16 m2 + 4 g + i (i = 0..3) of site p.  Their
+// tanh, weighted by w3, summed over the lane's eight neurons and across the four lane groups
+// (two butterfly exchanges), give y of site p in every group; the lane of group nb keeps it.
+// Float32 accuracy from half-precision matrix cores: every operand is split into a
+// half-precision head and tail (x = hi + lo, lo = half(x - hi)), and hi*hi + hi*lo + lo*hi is
+// accumulated in float32 (three MFMAs per product; the dropped lo*lo is below 2^-22 of it).
+// The per-lane weight halves (32 halves per lane) and the per-group first-layer, bias and
+// output weights (32 floats per group, first layer with this launch's f2 folded in) sit in
+// LDS, filled once per block.  tanh comes from the hardware exp2 and reciprocal.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float tanh_hw(float x)
+{
+    // tanh(x) = 1 - 2 / (e^{2x} + 1); e^{2x} = 2^{2x log2 e}: +-1 exactly for |x| large
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+}
+
+struct MlpMfmaRadial {
     const float* P;
     FeatureMap fm;
-    float f2;   // temperature feature of this launch
-    float g0v;  // g0(f2)
-    __device__ __forceinline__ float operator()(float rho) const
+    float f2, g0v;
+    float* cst;  // this block's constants in LDS (block_init)
+    static constexpr bool kEarly = false;
+    static constexpr bool kBatch = true;
+    static constexpr int kLaneFloats = 64 * 16;              // 32 weight halves per lane
+    static constexpr int kLdsFloats = kLaneFloats + 4 * 32;  // + per-group constants
+    struct Pre {
+        float f1, y;
+    };
+
+    __device__ __forceinline__ void block_init(float* l)
     {
-        const float x1 = fm.f1(rho);
-        return fm.vprime(rho, fmaf(x1, mlp_eval(P, x1, f2), g0v));
+        cst = l;
+        const int nt = (int)(blockDim.x * blockDim.y);
+        const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        _Float16* wh = reinterpret_cast<_Float16*>(l);
+        for (int e = tid; e < 2 * kLaneFloats; e += nt) {
+            // lane's halves: [m2][head, tail][j]
+            const int lane = e >> 5, q = e & 31;
+            const int g = lane >> 4, p = lane & 15, m2 = q >> 4, part = (q >> 3) & 1, j = q & 7;
+            const float w = P[OW2 + H * (16 * m2 + p) + 8 * g + j];
+            const _Float16 hi = (_Float16)w;
+            wh[e] = part ? (_Float16)(w - (float)hi) : hi;
+        }
+        for (int e = tid; e < 4 * 32; e += nt) {
+            const int g = e >> 5, c = e & 31, q = c & 7;
+            const int n1 = 8 * g + q;                          // first-layer neuron
+            const int n2 = 16 * (q >> 2) + 4 * g + (q & 3);    // second-layer neuron
+            float v;
+            if (c < 8) v = P[OW1 + 2 * n1];
+            else if (c < 16) v = fmaf(P[OW1 + 2 * n1 + 1], f2, P[OB1 + n1]);
+            else if (c < 24) v = P[OB2 + n2];
+            else v = P[OW3 + n2];
+            l[kLaneFloats + e] = v;
+        }
+        __syncthreads();
+    }
+
+    template <int TY>
+    __device__ __forceinline__ void batch(const float (&rho1)[TY], const float (&rho2)[TY],
+                                          Pre (&P1)[TY], Pre (&P2)[TY]) const
+    {
+        const int l = (int)threadIdx.x, g = l >> 4, p = l & 15;
+        const f16x8* wl = reinterpret_cast<const f16x8*>(cst) + 4 * l;
+        const f32x4* gc = reinterpret_cast<const f32x4*>(cst + kLaneFloats + g * 32);
+        const f16x8 ah0 = wl[0], al0 = wl[1], ah1 = wl[2], al1 = wl[3];
+        const f32x4 a10 = gc[0], a11 = gc[1], c10 = gc[2], c11 = gc[3];
+        const f32x4 b2a = gc[4], b2b = gc[5], w3a = gc[6], w3b = gc[7];
+        const float b3 = P[OB3];
+        float x1[2 * TY], y[2 * TY];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            x1[r] = fm.f1(rho1[r]);
+            x1[TY + r] = fm.f1(rho2[r]);
+        }
+#pragma unroll
+        for (int e = 0; e < 2 * TY; ++e) {
+            float ye = 0.0f;
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const float xf = __shfl(x1[e], nb * 16 + p);
+                f16x8 bh, bl;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const float h = tanh_hw(j < 4 ? fmaf(a10[j & 3], xf, c10[j & 3])
+                                                  : fmaf(a11[j & 3], xf, c11[j & 3]));
+                    const _Float16 hh = (_Float16)h;
+                    bh[j] = hh;
+                    bl[j] = (_Float16)(h - (float)hh);
+                }
+                f32x4 z0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah0, bh, b2a, 0, 0, 0);
+                f32x4 z1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah1, bh, b2b, 0, 0, 0);
+                z0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah0, bl, z0, 0, 0, 0);
+                z1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah1, bl, z1, 0, 0, 0);
+                z0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al0, bh, z0, 0, 0, 0);
+                z1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al1, bh, z1, 0, 0, 0);
+                float part = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    part = fmaf(w3a[i], tanh_hw(z0[i]), part);
+                    part = fmaf(w3b[i], tanh_hw(z1[i]), part);
+                }
+                part += __shfl_xor(part, 16);
+                part += __shfl_xor(part, 32);
+                ye = g == nb ? part : ye;
+            }
+            y[e] = ye + b3;
+        }
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            P1[r] = Pre{x1[r], y[r]};
+            P2[r] = Pre{x1[TY + r], y[TY + r]};
+        }
+    }
+
+    __device__ __forceinline__ float eval(const Pre& q, float rho) const
+    {
+        return fm.vprime(rho, fmaf(q.f1, q.y, g0v));
     }
 };
 
@@ -453,20 +569,40 @@ int launch_complex_mlp(const float* phi1, const float* phi2, const float* pi1, c
                 "rk2 c32 mlp: NULL pointer");
     TIP_REQUIRE(phi1 != o1 && phi2 != o2 && pi1 != o3 && pi2 != o4,
                 "rk2 c32 mlp: output must not alias input");
-    MlpForce f;
-    f.P = mlp_params;
-    f.fm = make_fm(fm);
-    f.f2 = (float)((T - fm->T_mid) / fm->T_half);
-    f.g0v = f.fm.g0(f.f2);
-    StepGeometry g;
+    // one y row per thread: the network's constants and the matrix-core operands need the
+    // registers a second row would take
+    const RowShape rs = row_shape(ny, nz, 1, 8);
+    TIP_REQUIRE(rs.ok, "rk2 c32 mlp: the inline-network step needs nz <= 512");
+    MlpMfmaRadial rad;
+    rad.P = mlp_params;
+    rad.fm = make_fm(fm);
+    rad.f2 = (float)((T - fm->T_mid) / fm->T_half);
+    rad.g0v = rad.fm.g0(rad.f2);
+    rad.cst = nullptr;
     TileGrid tg;
-    int nb;
-    StepGeometry req = {1, 0, 1, TIP_KERNEL_FLAT};
-    rc = resolve_geometry(&req, nx, ny, nz, 1, 0, g, tg, nb);
-    if (rc) return rc;
-    hipLaunchKernelGGL((k_rk2_complex<float, MlpForce, 1>), dim3(nb), dim3(64, 1), 0, st, phi1,
-                       phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, f, zn, (int)nx, (int)ny,
-                       (int)nz, tg, noise_seed, hash, vac, esc2);
+    tg.tiles = (int)((ny + (int64_t)rs.G - 1) / (int64_t)rs.G);
+    int64_t cx = (nx * tg.tiles) / 2048;
+    cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
+    tg.cx = (int)cx;
+    tg.chunks = (int)((nx + cx - 1) / cx);
+    const int64_t nb64 = (int64_t)tg.tiles * tg.chunks;
+    TIP_REQUIRE(nb64 < (int64_t(1) << 31), "rk2 c32 mlp: too many blocks");
+    tg.xcd_remap = nb64 % 8 == 0 ? 1 : 0;
+    const size_t lds =
+        ((size_t)rs.G * (64 * rs.W) * 9 + MlpMfmaRadial::kLdsFloats) * sizeof(float);
+    const dim3 blk(64, rs.W * rs.G);
+#define TIP_LAUNCH_MI(NM, CNT)                                                                 \
+    hipLaunchKernelGGL((k_rk2_complex_rows<1, NM, MlpMfmaRadial, CNT>), dim3((int)nb64), blk,   \
+                       lds, st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, nullptr, nullptr, c, rad, \
+                       zn, (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+    if (hash) {
+        if (vac) TIP_LAUNCH_MI(NOISE_HASH, true);
+        else TIP_LAUNCH_MI(NOISE_HASH, false);
+    } else {
+        if (vac) TIP_LAUNCH_MI(NOISE_NONE, true);
+        else TIP_LAUNCH_MI(NOISE_NONE, false);
+    }
+#undef TIP_LAUNCH_MI
     TIP_HIP(hipGetLastError());
     return TIP_OK;
 }
