@@ -417,6 +417,46 @@ def test_complex_lattice_geometries():
             assert all(np.array_equal(a, b) for a, b in zip(o, ref)), geo
 
 
+@pytest.mark.parametrize("shape", [(6, 16, 250), (5, 12, 256), (4, 10, 128)])
+def test_complex_shared_halo_rows(shape):
+    """Rows of two or four waves run two row groups per block that read their inner y-halo
+    rows from each other's LDS staging (one, two and four rows per thread, x chunks that do
+    and do not divide nx, ny that does and does not divide into pairs of tiles): identical
+    fields to the unshared form's per-site arithmetic, the flat tile within float32
+    rounding; Z_N on."""
+    from tip import potential as P
+    from tip.lattice import Lattice, KERNEL_FLAT, KERNEL_ROWS
+    rng = np.random.default_rng(sum(shape))
+    init = {k: (rng.normal(size=shape) * 2e3).astype(np.float32) for k in ("phi1", "phi2")}
+    init.update({k: (rng.normal(size=shape) * 1e2).astype(np.float32) for k in ("pi1", "pi2")})
+    t, tmin, dinv = P.build_vprime_table(6770.0, 0.0, 1e5, 4096, P.ModelParams.complex())
+    kw = dict(dt=0.25, dx=1.0, eta_eff=1.0, mu=1000.0, noise_scale=300.0)
+    outs = {}
+    geos = [(1, 0, 1, KERNEL_ROWS), (2, 3, 1, KERNEL_ROWS), (4, 0, 1, KERNEL_ROWS),
+            (4, 5, 0, KERNEL_ROWS), (1, 0, 1, KERNEL_FLAT)]
+    for geo in geos:
+        if geo[3] == KERNEL_ROWS and shape[1] % geo[0]:
+            continue
+        lat = Lattice(shape, np.float32, complex_field=True)
+        try:
+            for k, v in init.items():
+                lat.upload(k, v)
+            lat.set_table(t, tmin, dinv)
+            lat.set_zn(3.0, 1e8, 1.0)
+            lat.set_geometry(*geo)
+            lat.step(4, seed=8, step0=0, **kw)
+            outs[geo] = [lat.download(k) for k in ("phi1", "phi2", "pi1", "pi2")]
+        finally:
+            lat.close()
+    ref = outs[(1, 0, 1, KERNEL_ROWS)]
+    for geo, o in outs.items():
+        if geo[3] == KERNEL_FLAT:
+            for a, b in zip(o, ref):
+                assert rel(a.astype(np.float64), b.astype(np.float64)) < 1e-5, geo
+        else:
+            assert all(np.array_equal(a, b) for a, b in zip(o, ref)), geo
+
+
 # ----------------------------------------------------------------------------------------
 # winding, vacuum count
 # ----------------------------------------------------------------------------------------

@@ -890,7 +890,7 @@ __device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnPar
     complex_force_finish(rad, zn, a, b, rho, rad.fetch(rho), d1, d2);
 }
 
-template <int TY, int NM, typename Radial, bool CNT>
+template <int TY, int NM, typename Radial, bool CNT, bool HS = false>
 __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
@@ -915,6 +915,12 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     float* Lz = Lg + z * K;
     const float* Lzp = Lg + zp * K;
     const float* Lzm = Lg + zm * K;
+    // HS (two row groups per block, consecutive y tiles): group 0 loads the block's top halo
+    // row and reads its bottom halo from group 1's first staged row; group 1 loads the bottom
+    // halo and reads its top halo from group 0's last staged row
+    constexpr int NH = HS ? NF : 2 * NF;
+    const bool g0 = grp == 0;
+    const float* Lh = Lz + (g0 ? 64 * W * K : (TY - 1) - 64 * W * K);
 
     const int nyz = ny * nz;
     const size_t plane = (size_t)nyz * 4u;
@@ -951,17 +957,23 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 v2[f][r] = buf_load_s(at(F[f], xq), vo, so[r]);
             }
     }
-    float hA[2 * NF], hB[2 * NF] = {};  // [f] row y0-1, [NF + f] row y0+TY
+    // [f] row y0-1, [NF + f] row y0+TY; HS: [f] the group's one loaded halo row
+    float hA[NH], hB[NH] = {};
+    const uint32_t soh = g0 ? som : sop;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-        hA[f] = buf_load_s(at(F[f], x0), vo, som);
-        hA[NF + f] = buf_load_s(at(F[f], x0), vo, sop);
+        if constexpr (HS) {
+            hA[f] = buf_load_s(at(F[f], x0), vo, soh);
+        } else {
+            hA[f] = buf_load_s(at(F[f], x0), vo, som);
+            hA[NF + f] = buf_load_s(at(F[f], x0), vo, sop);
+        }
     }
     unsigned cnt = 0;
 
     auto plane_step = [&](auto bufc, int x, const float (&vm)[NF][TY], const float (&vc)[NF][TY],
                           const float (&vp)[NF][TY], float (&vn)[NF][TY],
-                          const float (&hc)[2 * NF], float (&hn)[2 * NF]) {
+                          const float (&hc)[NH], float (&hn)[NH]) {
         constexpr int b = decltype(bufc)::value;
         // Both stages' radii and table gathers of every row depend only on this plane's
         // fields: they are issued before the next plane's loads, so waiting for a gather
@@ -989,8 +1001,12 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             for (int f = 0; f < NF; ++f) {
 #pragma unroll
                 for (int r = 0; r < TY; ++r) vn[f][r] = buf_load_s(at(F[f], xb), vo, so[r]);
-                hn[f] = buf_load_s(at(F[f], xa), vo, som);
-                hn[NF + f] = buf_load_s(at(F[f], xa), vo, sop);
+                if constexpr (HS) {
+                    hn[f] = buf_load_s(at(F[f], xa), vo, soh);
+                } else {
+                    hn[f] = buf_load_s(at(F[f], xa), vo, som);
+                    hn[NF + f] = buf_load_s(at(F[f], xa), vo, sop);
+                }
             }
         }
         if constexpr (Radial::kBatch) {
@@ -1009,8 +1025,15 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             float sum[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const float yp = r + 1 < TY ? vc[f][r + 1 < TY ? r + 1 : r] : hc[NF + f];
-                const float ym = r > 0 ? vc[f][r > 0 ? r - 1 : 0] : hc[f];
+                float yp, ym;
+                if constexpr (HS) {
+                    const float hl = (r == 0 || r + 1 == TY) ? Lh[(b * NF + f) * TY] : 0.0f;
+                    yp = r + 1 < TY ? vc[f][r + 1 < TY ? r + 1 : r] : (g0 ? hl : hc[f]);
+                    ym = r > 0 ? vc[f][r > 0 ? r - 1 : 0] : (g0 ? hc[f] : hl);
+                } else {
+                    yp = r + 1 < TY ? vc[f][r + 1 < TY ? r + 1 : r] : hc[NF + f];
+                    ym = r > 0 ? vc[f][r > 0 ? r - 1 : 0] : hc[f];
+                }
                 const float zpv = Lzp[(b * NF + f) * TY + r], zmv = Lzm[(b * NF + f) * TY + r];
                 sum[f] = ((((vp[f][r] + vm[f][r]) + yp) + ym) + zpv) + zmv;
             }
@@ -1081,7 +1104,7 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                 v0[f][r] = v1[f][r]; v1[f][r] = v2[f][r]; v2[f][r] = v3[f][r];
             }
 #pragma unroll
-        for (int k = 0; k < 2 * NF; ++k) hA[k] = hB[k];
+        for (int k = 0; k < NH; ++k) hA[k] = hB[k];
     };
     if (x < x1) { plane_step(B0{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
     if (x < x1) { plane_step(B1{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
@@ -1090,7 +1113,8 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
 }
 
 // CNT: also count the sites with |phi_new|^2 > esc2 into *vac (separate instantiation).
-template <int TY, int NM, typename Radial, bool CNT>
+// HS: two row groups per block that share their inner y-halo rows through LDS.
+template <int TY, int NM, typename Radial, bool CNT, bool HS = false>
 __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
@@ -1107,7 +1131,7 @@ __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT>(
+    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT, HS>(
         phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, r, zn, nx, ny, nz, W, tp, x0,
         x1, noise_seed, esc2, lds_crows);
     if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);

@@ -246,18 +246,30 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
     TIP_REQUIRE(kind == TIP_KERNEL_AUTO || kind == TIP_KERNEL_ROWS || kind == TIP_KERNEL_FLAT,
                 "rk2 c32: unknown kernel kind %d", kind);
     if (kind != TIP_KERNEL_FLAT) {
-        // four fields per site: two y rows per thread at most by default (register budget)
-        int ty_req = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
-        if (!(geom && geom->by) && ty_req > 2) ty_req = 2;
-        const RowShape rs = row_shape(ny, nz, ty_req, 8);  // blocks of at most 512 threads
+        // Rows of two or four waves (nz in (64, 256]) run two row groups per block that share
+        // their inner y-halo rows through LDS (HS) when ny divides into the pairs of tiles;
+        // then four y rows per thread by default on large lattices (one 8-wave block per CU:
+        // 134 -> 128 us at 256^3 against 140 -> 134 us for 2 rows and no sharing, measured
+        // early / after the roll); otherwise at most two rows per thread (register budget).
+        const int w_row = (int)((nz + 63) / 64);
+        const int ty_def = default_rows_per_thread(nx, ny, nz);
+        int ty_req = (geom && geom->by) ? geom->by : ty_def;
+        const bool hs_shape = w_row == 2 || w_row == 4;
+        if (!(geom && geom->by) && ty_req > 2 && !(hs_shape && ny % (2 * ty_req) == 0))
+            ty_req = 2;
+        RowShape rs = row_shape(ny, nz, ty_req, 8);  // blocks of at most 512 threads
+        const bool hs = rs.ok && hs_shape && ny % (2 * rs.TY) == 0;
+        if (hs) rs.G = 2;
         TIP_REQUIRE(rs.ok || kind == TIP_KERNEL_AUTO,
                     "rk2 c32: row kernel needs nz <= 512 and rows per thread (1, 2, 4) dividing "
                     "ny (got %d rows, ny = %lld)", ty_req, (long long)ny);
         if (rs.ok) {
             TileGrid tg;
             tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
-            // about 2048 blocks (measured best at 256^3: 16 planes per block with TY = 2)
-            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 2048;
+            // about 2048 blocks (16 planes per block at 256^3 with TY = 2), or one block per
+            // CU for the shared-halo form with four rows per thread (32 planes at 256^3)
+            const int64_t target = (hs && rs.TY == 4) ? 256 : 2048;
+            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / target;
             if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
             TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
             tg.cx = (int)cx;
@@ -270,10 +282,17 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             const dim3 blk(64, rs.W * rs.G);
             TableRadial t32;
             t32.t = tab.as<false>();
-#define TIP_LAUNCH_CROWS_C(TYV, NM, CNT)                                                      \
-    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT>), dim3(nbr), blk, lds, st, \
-                       phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn,        \
+#define TIP_LAUNCH_CROWS_H(TYV, NM, CNT, HSV)                                                 \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT, HSV>), dim3(nbr), blk, lds, \
+                       st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn,      \
                        (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_CROWS_C(TYV, NM, CNT)          \
+    do {                                          \
+        if (hs)                                   \
+            TIP_LAUNCH_CROWS_H(TYV, NM, CNT, true); \
+        else                                      \
+            TIP_LAUNCH_CROWS_H(TYV, NM, CNT, false);\
+    } while (0)
 #define TIP_LAUNCH_CROWS(TYV, NM)              \
     do {                                       \
         if (vac)                               \
@@ -295,6 +314,7 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
 #undef TIP_LAUNCH_CROWS_NM
 #undef TIP_LAUNCH_CROWS
 #undef TIP_LAUNCH_CROWS_C
+#undef TIP_LAUNCH_CROWS_H
             TIP_HIP(hipGetLastError());
             return TIP_OK;
         }
