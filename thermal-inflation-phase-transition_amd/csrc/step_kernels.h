@@ -79,6 +79,42 @@ __device__ __forceinline__ unsigned wave_count(bool p)
 // quarter of a 64 x k tile idle).  Lanes past the plane compute on site 0 and store nothing,
 // so loads need no predication.  Returns the wave's count of sites with |phi_new| > esc when
 // CNT is set (all lanes of every wave call it with the same x0 and x1).
+// One site of the RK2 step in the reference's operation order (pass 1 :1912-1933, pass 2
+// :1945-1966): the six neighbours' phi_t = phi + dt/2 pi are formed from (phi, pi), so both
+// passes read only the step's input.  dV = V'(pc), dV2 = V'(ftc), ftc = pc + dt/2 qc (computed
+// by the caller for its table lookup with the same expression).
+template <typename R>
+__device__ __forceinline__ void rk2_site(const StepScalars<R>& c, R pc, R qc, R pp, R qp, R pm,
+                                         R qm, R pyp, R qyp, R pym, R qym, R pzp, R qzp, R pzm,
+                                         R qzm, R ftc, R dV, R dV2, R nz_, R& pn, R& qn)
+{
+    // Stage 1 (reference pass 1): k1 at (phi, pi); midpoint (phi_t, pi_t).
+    R s6 = pp + pm;
+    s6 = s6 + pyp;
+    s6 = s6 + pym;
+    s6 = s6 + pzp;
+    s6 = s6 + pzm;
+    R lap = (s6 - (R)6 * pc) * c.inv_dx2;
+    R k1 = c.inv_a2 * lap - c.eta * qc - dV * c.inv_mu2;
+    R ptc = qc + c.half_dt * k1;
+    // Stage 2 (reference pass 2): phi_t of the six neighbours is local to each of them.
+    R t_ip = pp + c.half_dt * qp;
+    R t_im = pm + c.half_dt * qm;
+    R t_jp = pyp + c.half_dt * qyp;
+    R t_jm = pym + c.half_dt * qym;
+    R t_kp = pzp + c.half_dt * qzp;
+    R t_km = pzm + c.half_dt * qzm;
+    R s6b = t_ip + t_im;
+    s6b = s6b + t_jp;
+    s6b = s6b + t_jm;
+    s6b = s6b + t_kp;
+    s6b = s6b + t_km;
+    R lap2 = (s6b - (R)6 * ftc) * c.inv_dx2;
+    R k2 = c.inv_a2 * lap2 - c.eta * ptc - dV2 * c.inv_mu2;
+    pn = pc + c.dt * ptc;
+    qn = qc + (c.dt * k2 + nz_);
+}
+
 template <typename R, typename Tab, int BY, bool CNT>
 __device__ __forceinline__ unsigned rk2_real_tile(
     const R* __restrict__ phi, const R* __restrict__ pi, R* __restrict__ phi_out,
@@ -130,39 +166,14 @@ __device__ __forceinline__ unsigned rk2_real_tile(
         const R qyp = buf_load<R>(Qb, o_jp), qym = buf_load<R>(Qb, o_jm);
         const R qzp = buf_load<R>(Qb, o_kp), qzm = buf_load<R>(Qb, o_km);
 
-        // Stage 1 (reference pass 1): k1 at (phi, pi); midpoint (phi_t, pi_t).
-        R s6 = pp + pm;
-        s6 = s6 + pyp;
-        s6 = s6 + pym;
-        s6 = s6 + pzp;
-        s6 = s6 + pzm;
-        R lap = (s6 - (R)6 * pc) * c.inv_dx2;
-        R dV = tab(pc);
-        R k1 = c.inv_a2 * lap - c.eta * qc - dV * c.inv_mu2;
-        R ftc = pc + c.half_dt * qc;
-        R ptc = qc + c.half_dt * k1;
-
-        // Stage 2 (reference pass 2): phi_t of the six neighbours is local to each of them.
-        R t_ip = pp + c.half_dt * qp;
-        R t_im = pm + c.half_dt * qm;
-        R t_jp = pyp + c.half_dt * qyp;
-        R t_jm = pym + c.half_dt * qym;
-        R t_kp = pzp + c.half_dt * qzp;
-        R t_km = pzm + c.half_dt * qzm;
-        R s6b = t_ip + t_im;
-        s6b = s6b + t_jp;
-        s6b = s6b + t_jm;
-        s6b = s6b + t_kp;
-        s6b = s6b + t_km;
-        R lap2 = (s6b - (R)6 * ftc) * c.inv_dx2;
-        R dV2 = tab(ftc);
-        R k2 = c.inv_a2 * lap2 - c.eta * ptc - dV2 * c.inv_mu2;
-
+        const R ftc = pc + c.half_dt * qc;
+        const R dV = tab(pc), dV2 = tab(ftc);
         R nz_ = 0;
         if (noise) nz_ = buf_load<R>(at(NZ, x), o_c);
         else if (hash_noise) nz_ = c.ns * gauss_of((R)0, (colK + xK) ^ noise_seed, c.word_mask);
-        R pn = pc + c.dt * ptc;
-        R qn = qc + (c.dt * k2 + nz_);
+        R pn, qn;
+        rk2_site<R>(c, pc, qc, pp, qp, pm, qm, pyp, qyp, pym, qym, pzp, qzp, pzm, qzm, ftc, dV, dV2,
+                    nz_, pn, qn);
         if (act) {
             buf_store(pn, at(PO, x), o_c);
             buf_store(qn, at(QO, x), o_c);

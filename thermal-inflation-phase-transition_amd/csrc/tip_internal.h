@@ -220,21 +220,36 @@ struct TableF64 {
     double tmin, dinv;
     int last;  // npts - 2
     int odd;
-    __device__ __forceinline__ double operator()(double x) const
+    // fetch() issues the two table loads of x's segment, eval() finishes the interpolation
+    // (recomputing the index), so a caller can issue several sites' loads before it needs them.
+    __device__ __forceinline__ void index(double x, double& fidx, int& idx) const
     {
-        double sgn = 1.0;
-        if (odd && x < 0.0) { x = -x; sgn = -1.0; }
-        double fidx = (x - tmin) * dinv;
+        if (odd && x < 0.0) x = -x;
+        fidx = (x - tmin) * dinv;
         // Clamping fidx to [-1, last+1] before the conversion changes idx only where the
         // index clamp below would anyway, and keeps the conversion in range (NaN -> 0).
-        int idx = (int)fmin(fmax(fidx, -1.0), (double)last + 1.0);
+        idx = (int)fmin(fmax(fidx, -1.0), (double)last + 1.0);
         if (idx < 0) idx = 0;
         else if (idx > last) idx = last;
-        double frac = fidx - (double)idx;
-        double t0 = t[idx], t1 = t[idx + 1];
-        double v = t0 + frac * (t1 - t0);
+    }
+    __device__ __forceinline__ double2 fetch(double x) const
+    {
+        double fidx;
+        int idx;
+        index(x, fidx, idx);
+        return make_double2(t[idx], t[idx + 1]);
+    }
+    __device__ __forceinline__ double eval(double2 s, double x) const
+    {
+        const double sgn = (odd && x < 0.0) ? -1.0 : 1.0;
+        double fidx;
+        int idx;
+        index(x, fidx, idx);
+        const double frac = fidx - (double)idx;
+        const double v = s.x + frac * (s.y - s.x);
         return odd ? sgn * v : v;
     }
+    __device__ __forceinline__ double operator()(double x) const { return eval(fetch(x), x); }
 };
 
 // ODD is a compile-time flag so the even table pays nothing for the odd extension.  The
