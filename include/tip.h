@@ -519,7 +519,8 @@ int tip_mlp_vprime(const float *params, const tip_mlp_features *fm, const float 
 int tip_mlp_tabulate(const float *params, const tip_mlp_features *fm, double T, double x_min,
                      double x_max, int64_t npts, double *table, void *stream);
 /* Complex fused RK2 step with V'(rho, T) evaluated by the MLP at every site and stage instead
- * of the table (the table entry point is tip_rk2_fused_c32). */
+ * of the table (the table entry point is tip_rk2_fused_c32): row tiles with the 32 x 32 layer
+ * on the matrix cores (half-precision head/tail products accumulated in float32); nz <= 512. */
 int tip_rk2_fused_c32_mlp(const float *phi1, const float *phi2, const float *pi1,
                           const float *pi2, float *phi1_out, float *phi2_out, float *pi1_out,
                           float *pi2_out, double dt, double dx, double eta_eff, double mu,
