@@ -147,6 +147,21 @@ def test_f64_hash_noise_ragged_shapes(shape):
     assert rel(g, a) <= 1e-12 and rel(h, b) <= 1e-12
 
 
+@pytest.mark.parametrize("shape", [(128, 256, 256), (128, 258, 260)])
+def test_f64_two_rows_per_thread_against_oracle(shape):
+    """Lattices of 2^23 sites and more run the float64 row tiles with two y rows per thread
+    (and, at nz = 260, five-wave rows ending in a wave of four active lanes): one step with
+    inline hash noise from a random field within 1e-12 of the oracle."""
+    rng = np.random.default_rng(shape[2])
+    phi0 = rng.normal(size=shape) * 3e3
+    pi0 = rng.normal(size=shape) * 1e3
+    t, tmin, dinv = real_table(1024)
+    prm = real_params()
+    a, b = O.run_real(phi0, pi0, 1, prm, t, tmin, dinv, odd=True, seed=5)
+    g, h = _run_gpu_real(phi0, pi0, 1, prm, t, tmin, dinv, True, np.float64, seed=5)
+    assert rel(g, a) <= 1e-12 and rel(h, b) <= 1e-12
+
+
 def test_f32_error_against_f64():
     """float32 kernel vs float64 oracle after 100 steps (64^3 real lattice): stated bound 1e-4."""
     rng = np.random.default_rng(7)
