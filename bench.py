@@ -272,7 +272,8 @@ def bench_extras(steps: int = 200) -> dict:
     """The other forms on one GPU: complex 256^3 float32 step and winding, the float64 drop-in
     real 256^3 step, the complex 512^3 step with the inline MLP surrogate (the 7350 -> 6770 K
     cooling workload's lattice; throughput does not depend on the weights, untrained ones are
-    used), the BAOAB and overdamped steps and the component labelling."""
+    used), the BAOAB and overdamped steps, the component labelling and the V' table
+    producer."""
     from tip import potential as P
     from tip import surrogate as S
     from tip.lattice import Lattice
@@ -355,6 +356,7 @@ def bench_extras(steps: int = 200) -> dict:
     log(f"complex 512^3 with the inline MLP: {us:.1f} us/step; per-step grid: {us_g:.1f} us/step")
     out.update(bench_other_integrators(steps))
     out["labels256"] = bench_labels()
+    out["vprime_table"] = bench_vprime_table()
     return out
 
 
@@ -404,6 +406,36 @@ def bench_other_integrators(steps: int = 200) -> dict:
                 "bytes_per_site_pass": b, "passes_per_step": passes,
                 "hbm_gbps_algorithmic": b * passes * n ** 3 / (us * 1e-6) / 1e9}
             log(f"{kind} {field} 256^3: {us:.1f} us/step")
+    return out
+
+
+def bench_vprime_table() -> dict:
+    """The V' table producer: tip_build_vprime_table on the device (8192 and 65536 points of
+    the real model's table at 7000 GeV, the rebuild a Hubble-on run does every few steps)
+    against the host producer (tip.potential.build_vprime_table, numpy float64)."""
+    from tip import potential as P
+    from tip import vprime_device as VD
+    from tip.device import synchronize
+
+    p = P.ModelParams.real()
+    vt = P.thermal_vev(7000.0, p)
+    out = {}
+    for npts in (8192, 65536):
+        tab, _, _ = VD.build_vprime_table(7000.0, 0.0, 1.5 * vt, npts, p)  # warm, spline upload
+        synchronize()
+        reps = 50
+        t0 = time.perf_counter()
+        for k in range(reps):
+            VD.build_vprime_table(7000.0 - 0.01 * k, 0.0, 1.5 * vt, npts, p, out=tab)
+        synchronize()
+        dev_us = 1e6 * (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        P.build_vprime_table(7000.0, 0.0, 1.5 * vt, npts, p)
+        host_us = 1e6 * (time.perf_counter() - t0)
+        tab.free()
+        out[str(npts)] = {"device_us_wall": dev_us, "host_numpy_us": host_us}
+        log(f"V' table {npts} points: device {dev_us:.1f} us (wall, incl. launch), host "
+            f"{host_us:.0f} us")
     return out
 
 

@@ -44,7 +44,7 @@ struct FeatureMap {
     __device__ __forceinline__ float vprime(float rho, float b) const
     {
         const float q = rho * inv_rho_cut;
-        const float th = out_scale * b / fmaf(q, q, 1.0f);
+        const float th = out_scale * b * __builtin_amdgcn_rcpf(fmaf(q, q, 1.0f));
         const double r = (double)rho;
         return (float)(r * (fma(lam * r, r, -m2) + (double)th));
     }
@@ -55,7 +55,7 @@ struct FeatureMap {
     __device__ __forceinline__ float f1(float rho) const
     {
         float a = rho * inv_rho_s;
-        return __logf(a + __builtin_sqrtf(a * a + 1.0f)) * inv_norm;
+        return __logf(a + __builtin_amdgcn_sqrtf(fmaf(a, a, 1.0f))) * inv_norm;
     }
     __device__ __forceinline__ float f2(float T) const { return (T - T_mid) * inv_T_half; }
 };
