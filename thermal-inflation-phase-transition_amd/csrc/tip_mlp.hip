@@ -114,15 +114,17 @@ __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, float x0,
 // accumulated in float32 (three MFMAs per product; the dropped lo*lo is below 2^-22 of it).
 // The per-lane weight halves (32 halves per lane) and the per-group first-layer, bias and
 // output weights (32 floats per group, first layer with this launch's f2 folded in) sit in
-// LDS, filled once per block.  tanh comes from the hardware exp2 and reciprocal.
+// LDS, filled once per block, pre-multiplied by 2 log2(e) where they feed a tanh.  tanh comes
+// from the hardware exp2 and reciprocal.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ float tanh_hw(float x)
+// 2 log2(e): the constants of both layers are stored multiplied by it, so the pre-activations
+// arrive as x' = 2 x log2(e) and tanh(x) = 1 - 2 / (2^x' + 1) (+-1 exactly for |x| large).
+constexpr float kTwoLog2e = 2.8853900817779268f;
+__device__ __forceinline__ float tanh_scaled(float xs)
 {
-    // tanh(x) = 1 - 2 / (e^{2x} + 1); e^{2x} = 2^{2x log2 e}: +-1 exactly for |x| large
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-    return fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(xs) + 1.0f), 1.0f);
 }
 
 struct MlpMfmaRadial {
@@ -148,7 +150,7 @@ struct MlpMfmaRadial {
             // lane's halves: [m2][head, tail][j]
             const int lane = e >> 5, q = e & 31;
             const int g = lane >> 4, p = lane & 15, m2 = q >> 4, part = (q >> 3) & 1, j = q & 7;
-            const float w = P[OW2 + H * (16 * m2 + p) + 8 * g + j];
+            const float w = P[OW2 + H * (16 * m2 + p) + 8 * g + j] * kTwoLog2e;
             const _Float16 hi = (_Float16)w;
             wh[e] = part ? (_Float16)(w - (float)hi) : hi;
         }
@@ -157,9 +159,9 @@ struct MlpMfmaRadial {
             const int n1 = 8 * g + q;                          // first-layer neuron
             const int n2 = 16 * (q >> 2) + 4 * g + (q & 3);    // second-layer neuron
             float v;
-            if (c < 8) v = P[OW1 + 2 * n1];
-            else if (c < 16) v = fmaf(P[OW1 + 2 * n1 + 1], f2, P[OB1 + n1]);
-            else if (c < 24) v = P[OB2 + n2];
+            if (c < 8) v = P[OW1 + 2 * n1] * kTwoLog2e;
+            else if (c < 16) v = fmaf(P[OW1 + 2 * n1 + 1], f2, P[OB1 + n1]) * kTwoLog2e;
+            else if (c < 24) v = P[OB2 + n2] * kTwoLog2e;
             else v = P[OW3 + n2];
             l[kLaneFloats + e] = v;
         }
@@ -192,8 +194,8 @@ struct MlpMfmaRadial {
                 f16x8 bh, bl;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    const float h = tanh_hw(j < 4 ? fmaf(a10[j & 3], xf, c10[j & 3])
-                                                  : fmaf(a11[j & 3], xf, c11[j & 3]));
+                    const float h = tanh_scaled(j < 4 ? fmaf(a10[j & 3], xf, c10[j & 3])
+                                                      : fmaf(a11[j & 3], xf, c11[j & 3]));
                     const _Float16 hh = (_Float16)h;
                     bh[j] = hh;
                     bl[j] = (_Float16)(h - (float)hh);
@@ -207,8 +209,8 @@ struct MlpMfmaRadial {
                 float part = 0.0f;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    part = fmaf(w3a[i], tanh_hw(z0[i]), part);
-                    part = fmaf(w3b[i], tanh_hw(z1[i]), part);
+                    part = fmaf(w3a[i], tanh_scaled(z0[i]), part);
+                    part = fmaf(w3b[i], tanh_scaled(z1[i]), part);
                 }
                 part += __shfl_xor(part, 16);
                 part += __shfl_xor(part, 32);
