@@ -365,6 +365,59 @@ def test_lattice_integrator_geometries(complex_field):
             assert rel(o[k], ref[k]) < 1e-6, (geo, op, k)
 
 
+@pytest.mark.parametrize("shape", [(6, 16, 250), (5, 12, 256), (4, 10, 128)])
+def test_complex_shared_halo_rows(shape):
+    """Complex rows of two or four waves run two row groups per block that read their inner
+    y-halo rows from each other's LDS staging (where ny divides into pairs of tiles): every
+    row geometry gives identical BAOAB and overdamped fields, the flat tile agrees to float32
+    rounding, and the default form matches the oracle."""
+    from tip.lattice import Lattice, KERNEL_FLAT, KERNEL_ROWS
+    f, q, tab, prm = complex_setup(shape, sum(shape))
+    init = {k: v.astype(np.float32) for k, v in
+            dict(phi1=f[0], phi2=f[1], pi1=q[0], pi2=q[1]).items()}
+    kw = dict(dt=prm["dt"], dx=prm["dx"], mu=prm["mu"], inv_a2=prm["inv_a2"],
+              noise_scale=prm["ns"], seed=3, step0=0)
+    out = {}
+    geos = [(1, 0, 1, KERNEL_ROWS), (2, 3, 1, KERNEL_ROWS), (4, 0, 1, KERNEL_ROWS),
+            (1, 0, 1, KERNEL_FLAT)]
+    for geo in geos:
+        if geo[3] == KERNEL_ROWS and shape[1] % geo[0]:
+            continue
+        for op in ("baoab", "overdamped"):
+            lat = Lattice(shape, np.float32, complex_field=True)
+            try:
+                for k, v in init.items():
+                    lat.upload(k, v)
+                lat.set_table(*tab, odd=False)
+                lat.set_zn(3.0, 1e8, 1.0)
+                lat.set_geometry(*geo)
+                if op == "baoab":
+                    lat.step_baoab(4, gamma=prm["eta"], **kw)
+                else:
+                    lat.step_overdamped(4, eta_eff=prm["eta"], **kw)
+                out[geo, op] = {k: lat.download(k) for k in init}
+            finally:
+                lat.close()
+    for (geo, op), o in out.items():
+        ref = out[(1, 0, 1, KERNEL_ROWS), op]
+        for k in init:
+            if geo[3] == KERNEL_FLAT:
+                assert rel(o[k].astype(np.float64), ref[k].astype(np.float64)) < 1e-6, (geo, op)
+            else:
+                assert np.array_equal(o[k], ref[k]), (geo, op, k)
+    # the default geometry (shared halos) through the drop-in against the oracle
+    from tip import kernels_complex as KC
+    a, b = (x.astype(np.float32) for x in f)
+    f0 = [f[0].astype(np.float32).astype(np.float64), f[1].astype(np.float32).astype(np.float64)]
+    q0 = [q[0].astype(np.float64), q[1].astype(np.float64)]
+    for n in range(3):
+        KC.overdamped_euler_step_table(a, b, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
+                                       prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]),
+                                       prm["ns"], O.noise_seed(n), zn=(3.0, 1e8, 1.0))
+    (ra, rb), _ = oracle_complex("overdamped", f0, q0, tab, prm, (3.0, 1e8, 1.0), 3)
+    assert rel(a.astype(np.float64), ra) < 2e-5 and rel(b.astype(np.float64), rb) < 2e-5
+
+
 def test_row_tile_fallbacks_and_errors():
     """A shape the row tile cannot take falls back to the flat tile under the default geometry
     (complex nz > 512) and agrees with the oracle; requesting the row kernel for such a shape

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64 * BY) void k_langevin_complex(
 // two halo rows, x neighbours the register window (x-1, x, x+1, and x+2 in flight).  Loads
 // per site: phi at x+2, pi at x+1 (BAOAB) and 2/TY halo rows per field.
 // ---------------------------------------------------------------------------------------
-template <int NF, int TY, int OP, bool ODD>
+template <int NF, int TY, int OP, bool ODD, bool HS = false>
 __device__ __forceinline__ void lv_rows_tile(
     const float* const (&phi)[NF], const float* const (&pi)[NF], float* const (&phi_out)[NF],
     float* const (&pi_out)[NF], const LangevinScalars<float>& c, const TableF32<ODD>& tab,
@@ -263,6 +263,11 @@ __device__ __forceinline__ void lv_rows_tile(
     float* Lz = Lg + z * K;
     const float* Lzp = Lg + zp * K;
     const float* Lzm = Lg + zm * K;
+    // HS (two row groups per block holding consecutive y tiles): each group loads one halo row
+    // of phi (group 0 the block's top one, group 1 the bottom one) and reads the other from the
+    // neighbour group's staged rows
+    const bool g0 = grp == 0;
+    const float* Lh = Lz + (g0 ? 64 * W * K : (TY - 1) - 64 * W * K);
 
     const int nyz = ny * nz;
     const size_t plane = (size_t)nyz * 4u;
@@ -285,6 +290,7 @@ __device__ __forceinline__ void lv_rows_tile(
     }
     const uint32_t offm = (uint32_t)((yb > 0 ? yb - 1 : ny - 1) * nz + zc) * 4u;
     const uint32_t offp = (uint32_t)((yb + TY < ny ? yb + TY : 0) * nz + zc) * 4u;
+    const uint32_t offh = g0 ? offm : offp;  // HS: the group's one loaded halo row
     const int64_t col0 = (int64_t)yb * nz + zc;
 
     // phi window (x-1, x, x+1, x+2 in the four role slots), pi at x (and x+1 in flight), halo
@@ -306,8 +312,13 @@ __device__ __forceinline__ void lv_rows_tile(
                 m0[f][r] = MOM ? buf_load<float>(at(pi[f], x0), off[r]) : 0.0f;
                 m1[f][r] = 0.0f;
             }
-            hA[f][0] = buf_load<float>(at(phi[f], x0), offm);
-            hA[f][1] = buf_load<float>(at(phi[f], x0), offp);
+            if constexpr (HS) {
+                hA[f][0] = buf_load<float>(at(phi[f], x0), offh);
+                hA[f][1] = 0.0f;
+            } else {
+                hA[f][0] = buf_load<float>(at(phi[f], x0), offm);
+                hA[f][1] = buf_load<float>(at(phi[f], x0), offp);
+            }
             hB[f][0] = hB[f][1] = 0.0f;
         }
     }
@@ -333,7 +344,7 @@ __device__ __forceinline__ void lv_rows_tile(
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int r = 0; r < TY; ++r) Lz[(b * NF + f) * TY + r] = pc[f][r];
-        {
+        if (x + 1 < x1) {  // the chunk's last plane has no next plane
             const int xa = x + 1 < nx ? x + 1 : 0;
             const int xb = xa + 1 < nx ? xa + 1 : 0;
 #pragma unroll
@@ -343,8 +354,12 @@ __device__ __forceinline__ void lv_rows_tile(
                     pn[f][r] = buf_load<float>(at(phi[f], xb), off[r]);
                     if (MOM) qn[f][r] = buf_load<float>(at(pi[f], xa), off[r]);
                 }
-                hn[f][0] = buf_load<float>(at(phi[f], xa), offm);
-                hn[f][1] = buf_load<float>(at(phi[f], xa), offp);
+                if constexpr (HS) {
+                    hn[f][0] = buf_load<float>(at(phi[f], xa), offh);
+                } else {
+                    hn[f][0] = buf_load<float>(at(phi[f], xa), offm);
+                    hn[f][1] = buf_load<float>(at(phi[f], xa), offp);
+                }
             }
         }
         __syncthreads();
@@ -355,8 +370,15 @@ __device__ __forceinline__ void lv_rows_tile(
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const int o = (b * NF + f) * TY + r;
-                const float yp = r + 1 < TY ? pc[f][r + 1 < TY ? r + 1 : r] : hc[f][1];
-                const float ym = r > 0 ? pc[f][r > 0 ? r - 1 : 0] : hc[f][0];
+                float yp, ym;
+                if constexpr (HS) {
+                    const float hl = (r == 0 || r + 1 == TY) ? Lh[(b * NF + f) * TY] : 0.0f;
+                    yp = r + 1 < TY ? pc[f][r + 1 < TY ? r + 1 : r] : (g0 ? hl : hc[f][0]);
+                    ym = r > 0 ? pc[f][r > 0 ? r - 1 : 0] : (g0 ? hc[f][0] : hl);
+                } else {
+                    yp = r + 1 < TY ? pc[f][r + 1 < TY ? r + 1 : r] : hc[f][1];
+                    ym = r > 0 ? pc[f][r > 0 ? r - 1 : 0] : hc[f][0];
+                }
                 float s1 = pp[f][r] + pm[f][r];
                 s1 = s1 + yp;
                 s1 = s1 + ym;
@@ -432,7 +454,7 @@ __device__ __forceinline__ void lv_rows_tile(
     if (x < x1) plane_step(B0{}, x, w0, w1, w2, w3, m0, m1, hA, hB);
 }
 
-template <int NF, int TY, int OP, bool ODD>
+template <int NF, int TY, int OP, bool ODD, bool HS = false>
 __global__ __launch_bounds__(NF == 1 ? 1024 : 512) void k_langevin_rows(
     const float* __restrict__ f0, const float* __restrict__ f1, const float* q0,
     const float* q1, float* __restrict__ o0, float* __restrict__ o1, float* r0, float* r1,
@@ -449,15 +471,15 @@ __global__ __launch_bounds__(NF == 1 ? 1024 : 512) void k_langevin_rows(
         const float* const pi[1] = {q0};
         float* const po[1] = {o0};
         float* const qo[1] = {r0};
-        lv_rows_tile<1, TY, OP, ODD>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
-                                      noise_seed, hash, lds_lv);
+        lv_rows_tile<1, TY, OP, ODD, HS>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0,
+                                          x1, noise_seed, hash, lds_lv);
     } else {
         const float* const phi[2] = {f0, f1};
         const float* const pi[2] = {q0, q1};
         float* const po[2] = {o0, o1};
         float* const qo[2] = {r0, r1};
-        lv_rows_tile<2, TY, OP, ODD>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0, x1,
-                                      noise_seed, hash, lds_lv);
+        lv_rows_tile<2, TY, OP, ODD, HS>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0,
+                                          x1, noise_seed, hash, lds_lv);
     }
 }
 

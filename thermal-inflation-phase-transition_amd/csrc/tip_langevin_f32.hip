@@ -19,8 +19,12 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
     launched = false;
     int ty = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
     if (NF == 2 && !(geom && geom->by) && ty > 2) ty = 2;
-    const RowShape rs = row_shape(ny, nz, ty, NF == 1 ? 16 : 8);
+    RowShape rs = row_shape(ny, nz, ty, NF == 1 ? 16 : 8);
     if (!rs.ok) return TIP_OK;
+    // complex rows of two or four waves: two row groups per block share their inner y-halo
+    // rows through LDS when ny divides into pairs of tiles
+    const bool hs = NF == 2 && (rs.W == 2 || rs.W == 4) && ny % (2 * rs.TY) == 0;
+    if (hs) rs.G = 2;
     TileGrid tg;
     tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
     int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
@@ -34,10 +38,18 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
     tg.xcd_remap = ((geom ? geom->xcd : 1) && nb % 8 == 0) ? 1 : 0;
     const size_t lds = (size_t)rs.G * (64 * rs.W) * (2 * NF * rs.TY + 1) * sizeof(float);
     const dim3 blk(64, rs.W * rs.G);
-#define TIP_LV_ROWS(TYV, OPV)                                                                   \
-    hipLaunchKernelGGL((k_langevin_rows<NF, TYV, OPV, ODD>), dim3(nb), blk, lds, st, f0, f1, q0, \
-                       q1, o0, o1, r0, r1, c, tab, zn, (int)nx, (int)ny, (int)nz, rs.W, tg,       \
+#define TIP_LV_ROWS_H(TYV, OPV, HSV)                                                           \
+    hipLaunchKernelGGL((k_langevin_rows<NF, TYV, OPV, ODD, NF == 2 && HSV>), dim3(nb), blk, lds, \
+                       st, f0, f1,                                                                  \
+                       q0, q1, o0, o1, r0, r1, c, tab, zn, (int)nx, (int)ny, (int)nz, rs.W, tg,    \
                        noise_seed, hash)
+#define TIP_LV_ROWS(TYV, OPV)                 \
+    do {                                      \
+        if (hs)                               \
+            TIP_LV_ROWS_H(TYV, OPV, true);    \
+        else                                  \
+            TIP_LV_ROWS_H(TYV, OPV, false);   \
+    } while (0)
 #define TIP_LV_ROWS_OP(TYV)                                              \
     switch (op) {                                                        \
         case LV_OVERDAMPED: TIP_LV_ROWS(TYV, LV_OVERDAMPED); break;      \
@@ -53,6 +65,7 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
     }
 #undef TIP_LV_ROWS_OP
 #undef TIP_LV_ROWS
+#undef TIP_LV_ROWS_H
     TIP_HIP(hipGetLastError());
     launched = true;
     return TIP_OK;
