@@ -73,11 +73,16 @@ def profile_record(name: str) -> dict:
 def host_description() -> dict:
     """CPU model, SMT state and core counts of the host the CPU baseline runs on."""
     model = None
+    ident = {}
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+            key = line.split(":", 1)[0].strip()
+            if key in ("model name", "cpu family", "model", "stepping", "cpu MHz") and \
+                    key not in ident:
+                ident[key] = line.split(":", 1)[1].strip()
+            if not line.strip() and ident:
+                break  # first processor's block only
+        model = ident.get("model name")
     except OSError:
         pass
     smt = None
@@ -89,7 +94,9 @@ def host_description() -> dict:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
         avail = os.cpu_count()
-    return {"cpu_model": model, "smt_active": smt, "nproc": avail,
+    return {"cpu_model": model, "cpu_family": ident.get("cpu family"),
+            "cpu_model_number": ident.get("model"), "cpu_stepping": ident.get("stepping"),
+            "cpu_mhz": ident.get("cpu MHz"), "smt_active": smt, "nproc": avail,
             "os_cpu_count": os.cpu_count()}
 
 
