@@ -311,13 +311,19 @@ def bench_extras(steps: int = 200) -> dict:
     for _ in range(5):
         strings, plaq = lat.winding_counts()
     w_ms = 1e3 * (time.perf_counter() - t0) / 5
+    # past the roll (steps 10 .. 10 + steps include it): the state of most of a long run
+    lat.step(3000 - 10 - steps, **kw)
+    us_settled = 1e3 * lat.step(steps, timed=True, **kw) / steps
     out["complex256"] = {"us_per_step": us, "site_updates_per_s": n ** 3 / (us * 1e-6),
                          "hbm_gbps_algorithmic": 32.0 * n ** 3 / (us * 1e-6) / 1e9,
-                         "bytes_per_site": 32.0, "winding_ms_wall": w_ms,
+                         "bytes_per_site": 32.0, "us_per_step_settled": us_settled,
+                         "hbm_gbps_algorithmic_settled": 32.0 * n ** 3 / (us_settled * 1e-6) / 1e9,
+                         "winding_ms_wall": w_ms,
                          "winding_gbps_algorithmic": 11.0 * n ** 3 / (w_ms * 1e-3) / 1e9,
                          "strings": strings}
     lat.close()
-    log(f"complex 256^3: {us:.1f} us/step, winding {w_ms:.2f} ms")
+    log(f"complex 256^3: {us:.1f} us/step (steps 10-{10 + steps}), {us_settled:.1f} us/step "
+        f"after 3000 steps, winding {w_ms:.2f} ms")
 
     table, tmin, dinv, prm = real_workload(n)
     lat = Lattice(shape, np.float64)
@@ -570,6 +576,21 @@ def main() -> int:
                      "traffic_kernel_us": prof.get("kernel_average_us")},
         "true_vacuum_fraction": frac_true,
     }
+    settled = None
+    if not a.no_extras and dtype == np.float32:
+        # the same lattice past its roll to the true vacuum (the state most of a 10^4-step run
+        # spends its time in): the step's table gathers then spread over many cache lines
+        done = a.warmup + a.steps
+        if done < 3000:
+            lat.step(3000 - done, **kw)
+        ms = lat.step(200, timed=True, **kw)
+        us = 1e3 * ms / 200
+        settled = {"after_steps": max(done, 3000), "us_per_step": us,
+                   "site_updates_per_s": sites / (us * 1e-6),
+                   "hbm_gbps_algorithmic": sites * bps / (us * 1e-6) / 1e9,
+                   "frac": sites * bps / (us * 1e-6) / 1e9 / HBM_PEAK_GBPS,
+                   "true_vacuum_fraction": lat.vacuum_fraction(1e4)}
+        log(f"real 256^3 after the roll (steps {settled['after_steps']}+200): {us:.1f} us/step")
     lat.close()
     if a.copy_calibration:
         from tip.device import DeviceArray, synchronize
@@ -588,6 +609,8 @@ def main() -> int:
         out["ensemble"] = bench_ensemble(modes=[m for m in a.ensemble_modes.split(",") if m])
     if not a.no_extras:
         out["extras"] = bench_extras()
+        if settled:
+            out["extras"]["real256_settled"] = settled
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shape, table, tmin, dinv, prm, a.cpu_seconds)
