@@ -1167,6 +1167,9 @@ struct RowShape {
 };
 RowShape row_shape(int64_t ny, int64_t nz, int ty, int max_w = 16);
 int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz);
+// Rows per thread and target block count of the real-field row kernels (RK2, BAOAB, overdamped).
+int real_rows_per_thread(int64_t nx, int64_t ny, int64_t nz);
+int64_t real_row_blocks(int64_t nx, int64_t ny, int64_t nz);
 StepGeometry step_geometry(int64_t nx, int64_t ny, int64_t nz, int complex_field, int f64);
 // Resolve a requested geometry (nullable = default) into the tile grid and the block count.
 int resolve_geometry(const StepGeometry* req, int64_t nx, int64_t ny, int64_t nz,

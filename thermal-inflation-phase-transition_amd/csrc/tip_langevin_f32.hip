@@ -17,7 +17,9 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
                    hipStream_t st, bool& launched)
 {
     launched = false;
-    int ty = (geom && geom->by) ? geom->by : default_rows_per_thread(nx, ny, nz);
+    int ty = (geom && geom->by) ? geom->by
+                                : (NF == 1 ? real_rows_per_thread(nx, ny, nz)
+                                           : default_rows_per_thread(nx, ny, nz));
     if (NF == 2 && !(geom && geom->by) && ty > 2) ty = 2;
     RowShape rs = row_shape(ny, nz, ty, NF == 1 ? 16 : 8);
     if (!rs.ok) return TIP_OK;
@@ -27,7 +29,8 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
     if (hs) rs.G = 2;
     TileGrid tg;
     tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
-    int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / 1024;
+    const int64_t target = NF == 1 ? real_row_blocks(nx, ny, nz) : 1024;
+    int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / target;
     if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
     TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
     tg.cx = (int)cx;

@@ -245,9 +245,11 @@ extern "C" int tip_lattice_set_geometry(tip_lattice* lat, int kernel, int rows,
     if (kernel == TIP_KERNEL_ROWS) {
         TIP_REQUIRE(lat->prec == 4,
                     "tip_lattice_set_geometry: the row kernel is the float32 step");
-        const RowShape rs = row_shape(
-            lat->ny, lat->nz, rows ? rows : default_rows_per_thread(lat->nx, lat->ny, lat->nz),
-            lat->kind == TIP_COMPLEX ? 8 : 16);
+        const int ty_def = lat->kind == TIP_COMPLEX
+                               ? default_rows_per_thread(lat->nx, lat->ny, lat->nz)
+                               : real_rows_per_thread(lat->nx, lat->ny, lat->nz);
+        const RowShape rs =
+            row_shape(lat->ny, lat->nz, rows ? rows : ty_def, lat->kind == TIP_COMPLEX ? 8 : 16);
         TIP_REQUIRE(rs.ok, "tip_lattice_set_geometry: row kernel needs nz <= 1024 (complex: 512) "
                            "and rows per thread (1, 2, 4) dividing ny");
         TIP_REQUIRE(planes_per_block >= 0 && planes_per_block <= 4096,

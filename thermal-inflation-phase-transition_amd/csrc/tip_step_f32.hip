@@ -90,17 +90,21 @@ int default_rows_per_thread(int64_t nx, int64_t ny, int64_t nz)
     return 1;
 }
 
-// The real float32 row kernel's default on large lattices: two rows per thread (79 registers,
+// The real-field row kernels' default on large lattices: two rows per thread (79 registers,
 // six waves per SIMD) and about 4096 blocks of 8 planes at 256^3 -- measured 46.8 us against
 // 50.0 us for four rows (124 registers, four waves per SIMD) in 1024 blocks, and 47.7-48.7 us
-// for 4, 6, 12 or 16 planes.
+// for 4, 6, 12 or 16 planes (BAOAB 45.2 against 49.3 us, overdamped 40.7 against 43.3 us).
 static bool real_large(int64_t nx, int64_t ny, int64_t nz)
 {
     return nx * ny * nz >= (int64_t(1) << 23);
 }
-static int real_rows_per_thread(int64_t nx, int64_t ny, int64_t nz)
+int real_rows_per_thread(int64_t nx, int64_t ny, int64_t nz)
 {
     return real_large(nx, ny, nz) && ny % 2 == 0 ? 2 : 1;
+}
+int64_t real_row_blocks(int64_t nx, int64_t ny, int64_t nz)
+{
+    return real_large(nx, ny, nz) ? 4096 : 1024;
 }
 
 RowShape row_shape(int64_t ny, int64_t nz, int ty, int max_w)
@@ -161,8 +165,8 @@ int launch_real_f32(const float* phi, const float* pi, float* phi_out, float* pi
             tg.tiles = (int)((ny + (int64_t)rs.G * rs.TY - 1) / ((int64_t)rs.G * rs.TY));
             // about 4096 blocks on large lattices (1024 on small ones), each marching enough
             // planes to amortise its window fill and halo rows
-            const int64_t target = real_large(nx, ny, nz) ? 4096 : 1024;
-            int64_t cx = (geom && geom->cx) ? geom->cx : (nx * tg.tiles) / target;
+            int64_t cx = (geom && geom->cx) ? geom->cx
+                                            : (nx * tg.tiles) / real_row_blocks(nx, ny, nz);
             if (!(geom && geom->cx)) cx = cx < 1 ? 1 : (cx > 32 ? 32 : cx);
             TIP_REQUIRE(cx >= 1 && cx <= 4096, "geometry: planes per block out of range");
             tg.cx = (int)cx;
