@@ -755,3 +755,40 @@ def test_full_size_256_noise_statistics():
     z = O.noise_field((1, 16, 256), 42, ns)
     np.testing.assert_allclose(pi[:4096], z.ravel(), rtol=0, atol=2e-6 * ns * 8)
     lat.close()
+
+
+@pytest.mark.parametrize("integrator", ["rk2", "baoab", "overdamped"])
+def test_full_size_256_default_geometry_bitwise(integrator):
+    """256^3 real float32: the default launch (two rows per thread, ~4096 XCD-ordered blocks of
+    8 planes) gives the same fields, bit for bit, as four rows per thread in 16-plane blocks
+    and as one row per thread without the XCD order, on a random field with hash noise."""
+    from tip.lattice import Lattice, KERNEL_AUTO, KERNEL_ROWS
+    n = 256
+    rng = np.random.default_rng(31)
+    phi0 = (rng.normal(size=(n, n, n)) * 2e3).astype(np.float32)
+    pi0 = (rng.normal(size=(n, n, n)) * 1e2).astype(np.float32)
+    t, tmin, dinv = real_table(8192)
+    prm = real_params()
+    lat = Lattice((n, n, n), np.float32)
+    lat.set_table(t, tmin, dinv, odd=True)
+    outs = []
+    try:
+        for geo in [(0, 0, 1, KERNEL_AUTO), (4, 16, 1, KERNEL_ROWS), (1, 8, 0, KERNEL_ROWS)]:
+            lat.set_geometry(*geo)
+            lat.upload("phi", phi0)
+            lat.upload("pi", pi0)
+            if integrator == "rk2":
+                lat.step(3, dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+                         noise_scale=prm["noise_scale"], seed=5, step0=0)
+            elif integrator == "baoab":
+                lat.step_baoab(3, dt=prm["dt"], dx=prm["dx"], gamma=prm["eta_eff"], mu=prm["mu"],
+                               noise_scale=30.0, seed=5, step0=0)
+            else:
+                lat.step_overdamped(3, dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"],
+                                    mu=prm["mu"], noise_scale=30.0, seed=5, step0=0)
+            outs.append((lat.download("phi"), lat.download("pi")))
+    finally:
+        lat.close()
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+    assert np.isfinite(outs[0][0]).all() and float(np.abs(outs[0][0] - phi0).max()) > 0.0
