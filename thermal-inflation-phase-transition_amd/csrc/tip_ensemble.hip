@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "step_kernels.h"
@@ -107,9 +108,12 @@ __global__ __launch_bounds__(256) void k_rk2_ensemble_rows(
     }
 }
 
-__global__ void k_advance(uint64_t* step_base, int n)
+__global__ void k_advance(uint64_t* step_base, uint64_t* step_base2, int n)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) step_base[0] += (uint64_t)n;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        step_base[0] += (uint64_t)n;
+        if (step_base2) step_base2[0] += (uint64_t)n;
+    }
 }
 
 // A run of work items launched together: items [first, last), threads per block, and whether
@@ -134,6 +138,10 @@ struct tip_ensemble {
     std::vector<int64_t> table_cap;
     std::vector<int> has_table, has_params;
     uint64_t* dev_step = nullptr;
+    // the second stream's copy of the step counter: graph replay advances each stream's
+    // chain of groups with its own counter, so the two chains never wait on each other
+    // between replays; every call leaves both equal to `step`
+    uint64_t* dev_step2 = nullptr;
     uint64_t step = 0;
     int parity = 0;
     hipStream_t stream = nullptr;
@@ -155,7 +163,8 @@ struct tip_ensemble {
     int nitems = 0;
     bool items_dirty = true;
     bool desc_dirty = true;
-    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;  // (nsteps, parity) -> exec
+    // (nsteps, parity, chain) -> exec; chain 1 is the second stream's groups
+    std::map<std::tuple<int64_t, int, int>, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -175,6 +184,7 @@ int release(tip_ensemble* E)
     if (E->dev_inst) (void)hipFree(E->dev_inst);
     if (E->dev_items) (void)hipFree(E->dev_items);
     if (E->dev_step) (void)hipFree(E->dev_step);
+    if (E->dev_step2) (void)hipFree(E->dev_step2);
     if (E->dev_vac) (void)hipFree(E->dev_vac);
     for (auto p : E->table64)
         if (p) (void)hipFree(p);
@@ -335,9 +345,9 @@ int build_items(tip_ensemble* E)
     return TIP_OK;
 }
 
-// One step of the items of group g on stream st.
+// One step of the items of group g on stream st, its noise seeds from the step counter ctr.
 int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g,
-                   hipStream_t st)
+                   hipStream_t st, const uint64_t* ctr)
 {
     const EnsItem* items = E->dev_items + g.first;
     const int nit = g.last - g.first;
@@ -353,7 +363,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g,
 #define TIP_LAUNCH_ENSR_C(TYV, ODD, DENSE)                                                     \
     hipLaunchKernelGGL((k_rk2_ensemble_rows<TYV, ODD, false, DENSE>), dim3(nb),              \
                        dim3(64, g.threads / 64), lds, st, E->dev_inst, items, nit, E->cx, \
-                       parity, E->dev_step, step_off, remap, vac, esc)
+                       parity, ctr, step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENSR(TYV, ODD)                 \
     do {                                          \
         if (g.dense)                              \
@@ -380,7 +390,7 @@ int launch_batched(tip_ensemble* E, int parity, int step_off, const EnsGroup& g,
     }
 #define TIP_LAUNCH_ENS(BYV, ODD)                                                                \
     hipLaunchKernelGGL((k_rk2_ensemble<BYV, ODD>), dim3(nb), dim3(64, BYV), 0, st,            \
-                       E->dev_inst, items, nit, E->cx, parity, E->dev_step,                    \
+                       E->dev_inst, items, nit, E->cx, parity, ctr,                            \
                        step_off, remap, vac, esc)
 #define TIP_LAUNCH_ENS_BY(ODD)                         \
     switch (E->flat_by) {                              \
@@ -415,7 +425,8 @@ int issue_steps(tip_ensemble* E, int64_t nsteps, int par0)
     for (size_t gi = 0; gi < E->groups.size(); ++gi) {
         const hipStream_t st = (two && (gi & 1)) ? E->stream2 : E->stream;
         for (int64_t s = 0; s < nsteps; ++s) {
-            int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, E->groups[gi], st);
+            int rc = launch_batched(E, par0 ^ (int)(s & 1), (int)s, E->groups[gi], st,
+                                    E->dev_step);
             if (rc) return rc;
         }
     }
@@ -444,22 +455,30 @@ int ready(tip_ensemble* E)
 // parity serves every full chunk.
 constexpr int64_t kGraphChunk = 16;
 
-// The instantiated graph of k steps starting at buffer parity `par` (captured on first use).
-int graph_for(tip_ensemble* E, int64_t k, int par, hipGraphExec_t* out)
+// The graph of k steps of one chain of groups starting at buffer parity `par`, captured on
+// first use: with `chains` = 2 chain c holds groups c, c + 2, ... (the groups the batched
+// schedule puts on stream c) and ends by advancing that stream's step counter; with one
+// chain it holds every group.
+int graph_for(tip_ensemble* E, int64_t k, int par, int chain, int chains, hipGraphExec_t* out)
 {
-    auto key = std::make_pair(k, par);
+    auto key = std::make_tuple(k, par, chain);
     auto itg = E->graphs.find(key);
     if (itg == E->graphs.end()) {
         hipGraph_t g = nullptr;
-        TIP_HIP(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-        int rc = issue_steps(E, k, par);
+        const hipStream_t st = chain ? E->stream2 : E->stream;
+        uint64_t* const ctr = chain ? E->dev_step2 : E->dev_step;
+        TIP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = TIP_OK;
+        for (size_t gi = (size_t)chain; gi < E->groups.size() && !rc; gi += (size_t)chains)
+            for (int64_t s = 0; s < k && !rc; ++s)
+                rc = launch_batched(E, par ^ (int)(s & 1), (int)s, E->groups[gi], st, ctr);
         if (rc) {
-            (void)hipStreamEndCapture(E->stream, &g);
+            (void)hipStreamEndCapture(st, &g);
             if (g) (void)hipGraphDestroy(g);
             return rc;
         }
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)k);
-        TIP_HIP(hipStreamEndCapture(E->stream, &g));
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, nullptr, (int)k);
+        TIP_HIP(hipStreamEndCapture(st, &g));
         hipGraphExec_t ex;
         hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -511,6 +530,8 @@ extern "C" int tip_ensemble_create(tip_ensemble** out, int n, const int64_t* dim
     if (e == hipSuccess) e = hipMalloc(&E->dev_inst, sizeof(EnsInst) * n);
     if (e == hipSuccess) e = hipMalloc(&E->dev_step, sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemsetAsync(E->dev_step, 0, sizeof(uint64_t), E->stream);
+    if (e == hipSuccess) e = hipMalloc(&E->dev_step2, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemsetAsync(E->dev_step2, 0, sizeof(uint64_t), E->stream);
     if (e == hipSuccess) e = hipMalloc(&E->dev_vac, sizeof(unsigned long long) * n);
     if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
     if (e != hipSuccess) {
@@ -679,28 +700,50 @@ extern "C" int tip_ensemble_step(tip_ensemble* E, int64_t nsteps, int mode, floa
             }
             E->parity ^= 1;
         }
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, E->dev_step2,
+                           (int)nsteps);
         TIP_HIP(hipGetLastError());
     } else if (mode == TIP_ENS_BATCHED || nsteps == 0) {
         rc = issue_steps(E, nsteps, E->parity);
         if (rc) return rc;
         if (nsteps & 1) E->parity ^= 1;
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, (int)nsteps);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step, E->dev_step2,
+                           (int)nsteps);
         TIP_HIP(hipGetLastError());
     } else {
-        // The graph holds a chunk of at most kGraphChunk steps (groups x chunk kernel nodes
-        // plus the counter bump) and is replayed; a call of n steps is n / chunk replays of
-        // the full chunk and one of the remainder.  Each replay advances the device step
-        // counter by its own chunk, so the noise seeds equal those of batched launches.
+        // Each chain of groups (one per stream) is a graph of a chunk of at most kGraphChunk
+        // steps (its groups x chunk kernel nodes plus its counter bump), replayed on its own
+        // stream; a call of n steps is n / chunk replays of the full chunk and one of the
+        // remainder.  The chains are forked from the ensemble's stream once per call and
+        // joined at its end, so successive replays of one chain follow each other without
+        // waiting for the other chain.  Each replay advances its chain's step counter by
+        // its chunk, so the noise seeds equal those of batched launches.
+        const bool two = E->concurrency == 2 && E->groups.size() > 1;
+        const int chains = two ? 2 : 1;
+        if (two) {
+            TIP_HIP(hipEventRecord(E->fork, E->stream));
+            TIP_HIP(hipStreamWaitEvent(E->stream2, E->fork, 0));
+        }
         int64_t left = nsteps;
         while (left > 0) {
             const int64_t k = left < kGraphChunk ? left : kGraphChunk;
-            hipGraphExec_t ex = nullptr;
-            rc = graph_for(E, k, E->parity, &ex);
-            if (rc) return rc;
-            TIP_HIP(hipGraphLaunch(ex, E->stream));
+            for (int c = 0; c < chains; ++c) {
+                hipGraphExec_t ex = nullptr;
+                rc = graph_for(E, k, E->parity, c, chains, &ex);
+                if (rc) return rc;
+                TIP_HIP(hipGraphLaunch(ex, c ? E->stream2 : E->stream));
+            }
             if (k & 1) E->parity ^= 1;
             left -= k;
+        }
+        if (two) {
+            TIP_HIP(hipEventRecord(E->join, E->stream2));
+            TIP_HIP(hipStreamWaitEvent(E->stream, E->join, 0));
+        } else {
+            // the second counter follows the first
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, E->stream, E->dev_step2,
+                               nullptr, (int)nsteps);
+            TIP_HIP(hipGetLastError());
         }
     }
     E->step += (uint64_t)nsteps;
@@ -740,6 +783,7 @@ extern "C" int tip_ensemble_set_step(tip_ensemble* E, uint64_t step)
     TIP_REQUIRE(E, "tip_ensemble_set_step: NULL");
     TIP_HIP(hipStreamSynchronize(E->stream));
     TIP_HIP(hipMemcpy(E->dev_step, &step, sizeof(uint64_t), hipMemcpyHostToDevice));
+    TIP_HIP(hipMemcpy(E->dev_step2, &step, sizeof(uint64_t), hipMemcpyHostToDevice));
     E->step = step;
     return TIP_OK;
 }
