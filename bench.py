@@ -275,6 +275,80 @@ def bench_ensemble(steps: int = 200, modes=("graph", "batched", "sequential")):
     return out
 
 
+def bench_config_runs() -> dict:
+    """The 256^3 workloads end to end at their own lengths, observables included: the real
+    field for 10^4 RK2 steps in float32 and in the float64 drop-in precision with the
+    true-vacuum fraction every 500 steps (the 8192-point table on [0, 1.5 phi_true] at 7000
+    GeV), and the complex field for 5 x 10^4 steps at 6770 GeV with the plaquette windings and
+    string count every 500 steps.  Rates count the observables' time too."""
+    from tip import potential as P
+    from tip.lattice import Lattice
+
+    n = 256
+    shape = (n, n, n)
+    sites = float(n ** 3)
+    out = {}
+    table, tmin, dinv, prm = real_workload(n)
+    kw = dict(dt=prm["dt"], dx=prm["dx"], eta_eff=prm["eta_eff"], mu=prm["mu"],
+              noise_scale=prm["noise_scale"])
+    for dtype, name in ((np.float32, "real256_f32_1e4"), (np.float64, "real256_f64_1e4")):
+        lat = Lattice(shape, dtype)
+        lat.upload("phi", 0.01 * np.random.default_rng(0).normal(size=shape))
+        lat.upload("pi", np.zeros(shape))
+        lat.set_table(table, tmin, dinv, odd=True)
+        total, report = 10000, 500
+        f_true = []
+        lat.step(1, **kw)  # first launch (module load) outside the clock
+        lat.upload("phi", 0.01 * np.random.default_rng(0).normal(size=shape))
+        lat.upload("pi", np.zeros(shape))
+        t0 = time.perf_counter()
+        for k in range(total // report):
+            lat.step(report, step0=k * report, **kw)
+            f_true.append(lat.vacuum_fraction(1e4))
+        wall = time.perf_counter() - t0
+        lat.close()
+        first = next((report * (i + 1) for i, f in enumerate(f_true) if f > 0.5), None)
+        out[name] = {"steps": total, "wall_s": wall, "ms_per_step": 1e3 * wall / total,
+                     "site_updates_per_s": sites * total / wall,
+                     "hbm_gbps_algorithmic": (16.0 if dtype == np.float32 else 32.0) * sites *
+                     total / wall / 1e9,
+                     "f_true_every_500": f_true, "first_step_f_true_above_half": first}
+        log(f"config run {name}: {1e3 * wall / total:.4f} ms/step over {total} steps with the "
+            f"vacuum fraction every {report}; f_true > 0.5 from step {first}")
+    mu, T0 = 1000.0, 6770.0
+    pc = P.ModelParams.complex()
+    vt = P.thermal_vev(T0, pc)
+    t, tmin, dinv = P.build_vprime_table(T0, 0.0, 1.5 * vt, 8192, pc)
+    dt, dx, eta = mu * 2.5e-4, mu * 1e-3, 1000.0 / mu
+    kwc = dict(dt=dt, dx=dx, eta_eff=eta, mu=mu, inv_a2=1.0,
+               noise_scale=P.noise_scale(eta, T0, dt, mu, 1e-3))
+    rng = np.random.default_rng(2)
+    th = 2 * np.pi * rng.random(shape)
+    r = 0.01 * np.abs(rng.normal(size=shape))
+    lat = Lattice(shape, np.float32, complex_field=True)
+    lat.set_table(t, tmin, dinv, odd=False)
+    lat.step(1, **kwc)
+    lat.upload("phi1", r * np.cos(th))
+    lat.upload("phi2", r * np.sin(th))
+    lat.upload("pi1", np.zeros(shape))
+    lat.upload("pi2", np.zeros(shape))
+    total, report = 50000, 500
+    strings = []
+    t0 = time.perf_counter()
+    for k in range(total // report):
+        lat.step(report, step0=k * report, **kwc)
+        strings.append(int(lat.winding_counts()[0]))
+    wall = time.perf_counter() - t0
+    lat.close()
+    out["complex256_5e4"] = {"steps": total, "wall_s": wall, "ms_per_step": 1e3 * wall / total,
+                             "site_updates_per_s": sites * total / wall,
+                             "hbm_gbps_algorithmic": 32.0 * sites * total / wall / 1e9,
+                             "strings_every_500": strings}
+    log(f"config run complex256_5e4: {1e3 * wall / total:.4f} ms/step over {total} steps with "
+        f"the windings every {report}; strings {strings[0]} -> {strings[-1]}")
+    return out
+
+
 def bench_extras(steps: int = 200) -> dict:
     """The other forms on one GPU: complex 256^3 float32 step and winding, the float64 drop-in
     real 256^3 step, the complex 512^3 step with the inline MLP surrogate (the 7350 -> 6770 K
@@ -498,6 +572,8 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the complex, float64 and surrogate throughput lines")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-config-runs", action="store_true",
+                    help="skip the end-to-end 256^3 runs at their own lengths (~10 s)")
     ap.add_argument("--ensemble-modes", default="graph,batched,sequential",
                     help="comma-separated subset of graph, batched, sequential")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -609,6 +685,8 @@ def main() -> int:
         out["ensemble"] = bench_ensemble(modes=[m for m in a.ensemble_modes.split(",") if m])
     if not a.no_extras:
         out["extras"] = bench_extras()
+        if not a.no_config_runs:
+            out["extras"]["config_runs"] = bench_config_runs()
         if settled:
             out["extras"]["real256_settled"] = settled
     if not a.no_cpu:
