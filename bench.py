@@ -275,7 +275,7 @@ def bench_ensemble(steps: int = 200, modes=("graph", "batched", "sequential")):
     return out
 
 
-def bench_config_runs() -> dict:
+def bench_config_runs(ensemble_settings=None) -> dict:
     """The 256^3 workloads end to end at their own lengths, observables included: the real
     field for 10^4 RK2 steps in float32 and in the float64 drop-in precision with the
     true-vacuum fraction every 500 steps (the 8192-point table on [0, 1.5 phi_true] at 7000
@@ -346,6 +346,21 @@ def bench_config_runs() -> dict:
                              "strings_every_500": strings}
     log(f"config run complex256_5e4: {1e3 * wall / total:.4f} ms/step over {total} steps with "
         f"the windings every {report}; strings {strings[0]} -> {strings[-1]}")
+    # the nucleation ensemble: 2 x 10^4 steps, per-instance vacuum fractions every 500 steps,
+    # graph replay with the launch settings the ensemble bench tuned (if it ran)
+    from tip.driver import nucleation_ensemble
+    geo, gb, conc = ensemble_settings or (None, None, None)
+    r = nucleation_ensemble(steps=20000, report=500, mode="graph", geometry=geo,
+                            group_bytes=gb, concurrency=conc)
+    nuc = [int(v) for v in r["nucleation_step"]]
+    out["nucleation_ensemble_2e4"] = {
+        "steps": 20000, "wall_s": r["elapsed_s"], "ms_per_step": 1e3 * r["elapsed_s"] / 20000,
+        "site_updates_per_s": r["site_updates_per_s"],
+        "hbm_gbps_algorithmic": 16.0 * r["site_updates_per_s"] / 1e9,
+        "nucleation_step": nuc, "instances_nucleated": sum(v >= 0 for v in nuc),
+        "launch_settings": [list(geo) if geo is not None else None, gb, conc]}
+    log(f"config run nucleation_ensemble_2e4: {1e3 * r['elapsed_s'] / 20000:.4f} ms/step, "
+        f"{sum(v >= 0 for v in nuc)}/64 nucleated by step {max(nuc)}")
     return out
 
 
@@ -686,7 +701,10 @@ def main() -> int:
     if not a.no_extras:
         out["extras"] = bench_extras()
         if not a.no_config_runs:
-            out["extras"]["config_runs"] = bench_config_runs()
+            ens = out.get("ensemble")
+            settings = ((ens["autotune_best"], ens["group_bytes"], ens["groups_at_once"])
+                        if ens else None)
+            out["extras"]["config_runs"] = bench_config_runs(settings)
         if settled:
             out["extras"]["real256_settled"] = settled
     if not a.no_cpu:

@@ -339,10 +339,13 @@ class Run:
 
 
 def nucleation_ensemble(steps: int = 20000, report: int = 500, esc: float = 1e4,
-                        frac: float = 0.5, mode: str = "graph", log=None):
+                        frac: float = 0.5, mode: str = "graph", log=None, geometry=None,
+                        group_bytes: Optional[int] = None, concurrency: Optional[int] = None):
     """The nucleation ensemble: 64 realisations (32 x 64^3, 24 x 96^3, 8 x 128^3), seeds 1..64,
     T0 ~ U[6770, 7350], per-instance 8192-point tables; returns each instance's nucleation step
-    (first report step with true-vacuum fraction > frac; -1 if none) and the fraction series."""
+    (first report step with true-vacuum fraction > frac; -1 if none) and the fraction series.
+    geometry / group_bytes / concurrency: the ensemble's launch settings (Ensemble.set_geometry,
+    set_schedule, set_concurrency; None keeps the defaults), e.g. an autotuner's choice."""
     from .ensemble import Ensemble, nucleation_ensemble_shapes
 
     shapes = nucleation_ensemble_shapes()
@@ -362,6 +365,12 @@ def nucleation_ensemble(steps: int = 20000, report: int = 500, esc: float = 1e4,
     for i, s in enumerate(shapes):
         ens.upload(i, "phi", 0.01 * r2.normal(size=s))
         ens.upload(i, "pi", np.zeros(s))
+    if geometry is not None:
+        ens.set_geometry(*geometry)
+    if group_bytes is not None:
+        ens.set_schedule(int(group_bytes))
+    if concurrency is not None:
+        ens.set_concurrency(int(concurrency))
     nuc = np.full(len(shapes), -1, dtype=np.int64)
     series = []
     t0 = time.perf_counter()
