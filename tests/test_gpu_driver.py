@@ -199,3 +199,30 @@ def test_driver_real_f64_other_integrators_replay(integrator):
         assert np.linalg.norm(pi_g - pi) <= 1e-12 * np.linalg.norm(pi)
     else:
         assert np.array_equal(pi_g, f0["pi"])
+
+
+@pytest.mark.parametrize("field,integrator", [("real", "rk2"), ("complex", "rk2"),
+                                              ("real", "baoab")])
+def test_range_every_chunks_equal_stepwise(field, integrator, monkeypatch):
+    """Constant parameters with the table range checked every 25 steps (steps in between sent
+    to the GPU in one call): the same fields, bit for bit, and the same table builds as the
+    reference's check at every step, on a run whose field stays inside the first table (a
+    table margin of at least 10^6 GeV keeps it there)."""
+    from tip import driver as D
+    monkeypatch.setattr(D, "TABLE_MARGIN_MIN", 1.0e6)
+    outs = []
+    for every in (1, 25):
+        cfg = D.RunConfig(field=field, Nx=24, Ny=20, Nz=64, Nt=100, T0=7000.0 if field == "real"
+                          else 6770.0, integrator=integrator, report=50, seed=5, quiet=True,
+                          table_size=2048, range_every=every)
+        run = D.Run(cfg)
+        try:
+            series = run.run()
+            names = ("phi1", "phi2", "pi1", "pi2") if field == "complex" else ("phi", "pi")
+            outs.append(([run.lat.download(k) for k in names], run.rebuilds, series))
+        finally:
+            run.close()
+    (a, ra, sa), (b, rb, sb) = outs
+    assert ra == rb == 1
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert sa["f_true"] == sb["f_true"] and sa["step"] == sb["step"] == [50, 100]

@@ -43,6 +43,7 @@ from .lattice import Lattice
 
 MU = 1000.0
 TABLE_MARGIN_FRAC = 1.0  # latticeSimeRescale_numba.py:3604
+TABLE_MARGIN_MIN = 20000.0  # latticeSimeRescale_numba.py:3662, latticeSimComplex_numba.py:4277
 CHECKPOINT_EVERY = 10    # latticeSimeRescale_numba.py:395
 
 
@@ -83,6 +84,10 @@ class RunConfig:
     surrogate_grid: int = 0  # 0: network inline per site; > 0: per-step grid of that many points
     divergence: float = 1e6  # |phi|max threshold of the reference's guard (0 disables)
     guard_every: int = 100000  # latticeSimeRescale_numba.py:3883
+    range_every: int = 1     # steps between checks of the field against the table's range (1:
+                             # every step, as the reference :3650-3672; > 1: one device min/max
+                             # per this many steps, and with constant parameters the steps in
+                             # between go to the GPU in one call)
     quiet: bool = False
     progress: int = 0        # log a progress line every this many steps (0 = off)
 
@@ -182,8 +187,10 @@ class Run:
             return 0.0, hi  # rho max
         return lo, hi
 
-    def maybe_rebuild(self, T: float) -> None:
+    def maybe_rebuild(self, T: float, n: int = 0) -> None:
         need = self.table_T is None or abs(T - self.table_T) / max(abs(T), 1.0) > 1e-4
+        if not need and n % max(self.cfg.range_every, 1) != 0:
+            return  # the field's range is checked every range_every steps
         lo, hi = self.field_range()
         if not (math.isfinite(lo) and math.isfinite(hi)):
             raise FloatingPointError("non-finite field")
@@ -192,7 +199,7 @@ class Run:
         if not need:
             return
         rng = max(hi - lo, 1.0)
-        margin = max(rng * TABLE_MARGIN_FRAC, 20000.0)
+        margin = max(rng * TABLE_MARGIN_FRAC, TABLE_MARGIN_MIN)
         t_lo = max(0.0, lo - margin) if self.cplx else lo - margin
         t_hi = hi + margin
         if self.cfg.table_source == "exact":
@@ -270,11 +277,13 @@ class Run:
 
     # -- main loop -----------------------------------------------------------------------
     def _chunk(self, n: int) -> int:
-        """Steps until the next report / guard / snapshot / progress boundary (surrogate path:
-        the temperature is linear in n inside a chunk, so one C call runs it)."""
+        """Steps until the next report / guard / snapshot / progress / range-check boundary (the
+        surrogate path: the temperature is linear in n inside a chunk; the table path with
+        constant parameters: nothing changes inside a chunk; either way one C call runs it)."""
         cfg = self.cfg
         k = cfg.Nt - n
-        for every in (cfg.report, cfg.guard_every, cfg.steps if cfg.outdir else 0, cfg.progress):
+        for every in (cfg.report, cfg.guard_every, cfg.steps if cfg.outdir else 0, cfg.progress,
+                      cfg.range_every if self.surrogate is None else 0):
             if every:
                 k = min(k, every - n % every)
         return max(k, 1)
@@ -305,15 +314,19 @@ class Run:
                 self.lat.step_mlp(1, self.surrogate, T_start=T, grid_points=cfg.surrogate_grid,
                                   **kw)
             else:
-                k = 1
-                self.maybe_rebuild(T)
+                # constant parameters (no expansion, no cooling, no Z_N switch ahead): the steps
+                # up to the next check, report or snapshot go to the GPU in one call
+                const = (self.hubble is None and cfg.cool_to is None and
+                         not (cfg.zn_order > 0 and cfg.zn_turn_on_T > 0 and self.zn_active < 0.5))
+                k = self._chunk(n) if const else 1
+                self.maybe_rebuild(T, n)
                 if self.integrator == "baoab":
                     kw["gamma"] = kw.pop("eta_eff")
-                    self.lat.step_baoab(1, **kw)
+                    self.lat.step_baoab(k, **kw)
                 elif self.integrator == "overdamped":
-                    self.lat.step_overdamped(1, **kw)
+                    self.lat.step_overdamped(k, **kw)
                 else:
-                    self.lat.step(1, **kw)
+                    self.lat.step(k, **kw)
             n += k
             T_now = T  # temperature of the last step taken
             if cfg.report and n % cfg.report == 0:
