@@ -1,27 +1,37 @@
-// bm_math.h -- float64 log and cos / sin for the hash Box-Muller draw, on the draw's own
-// domains (u1 in [1e-12, 1), the angle 2 pi u2 in [0, 2 pi)).
+// bm_math.h -- float64 log, sqrt and cos / sin for the hash Box-Muller draw, on the draw's own
+// domains (u1 in [1e-12, 1), -2 log u1 in (0, 56), the angle 2 pi u2 with u2 in [0, 1)).
 //
 // The generic device libm routines handle every argument (cos of any magnitude needs a
-// Payne-Hanek reduction); here the arguments are bounded, so each function is a short,
-// branch-free sequence:
+// Payne-Hanek reduction, sqrt and division scale for denormals and overflow); here the
+// arguments are bounded, so each function is a short, branch-free sequence:
 //
 //  * log(x), x a positive normal number: x = 2^e m with m in [sqrt(1/2), sqrt(2)),
 //    s = (m - 1) / (m + 1), log m = 2 atanh(s) = 2 s (1 + s^2/3 + s^4/5 + ...) (the atanh
 //    series; |s| <= 0.1716, the terms to s^22 leave < 1e-18 relative), and
 //    log x = e ln2_hi + (e ln2_lo + log m) with ln 2 split so that e ln2_hi is exact.
-//  * cos(x), sin(x) for 0 <= x < 2^20: k = nearest integer to x / (pi/2),
-//    r = x - k pi/2 by Cody and Waite's two-constant reduction (pi/2 = pio2_hi + pio2_lo,
-//    |r| <= pi/4), the Taylor series of sin r (to r^21) and cos r (to r^22) on |r| <= pi/4,
-//    and the quadrant k mod 4 selecting +-sin r / +-cos r.
+//    The quotient is f * y with y the hardware reciprocal of m + 1 after two Newton steps,
+//    then one residual correction q + (f - d q) y (Markstein's division; d in [1.7, 2.5]
+//    needs none of the generic routine's scaling).
+//  * sqrt(t), 0 < t < 2^64 (normal): y = hardware 1/sqrt(t), one coupled Newton step for
+//    g ~ sqrt(t) and h ~ 1/(2 sqrt(t)) (Goldschmidt), then two residual corrections
+//    g + (t - g^2) h.
+//  * cos(2 pi u) for u in [0, 1): with v = min(u, 1 - u) (exact for a u of 53 bits),
+//    cos(2 pi u) = 1 - 2 sin^2(pi v) and pi v in [0, pi/2], sin by its Taylor series to the
+//    x^21 term (the next term is below 1.2e-18 there).  For v = u, pi v is exactly half of
+//    the rounded 2 pi u (the two constants are one binary exponent apart).
+//  * cos(x), sin(x) together for 0 <= x < 2^20 (the complex field's pair): k = nearest integer
+//    to x / (pi/2), r = x - k pi/2 by Cody and Waite's two-constant reduction
+//    (pi/2 = pio2_hi + pio2_lo, |r| <= pi/4), the Taylor series of sin r (to r^21) and cos r
+//    (to r^22) on |r| <= pi/4, and the quadrant k mod 4 selecting +-sin r / +-cos r.
 //
-// Error: a few ulp for log over its domain and < 2e-16 absolute for cos / sin; the draw
-// z = sqrt(-2 log u1) cos(2 pi u2) then agrees with the host libm's to ~1e-15 of
+// Error: a few ulp for log and sqrt over their domains and < 6e-16 absolute for cos / sin;
+// the draw z = sqrt(-2 log u1) cos(2 pi u2) then agrees with the host libm's to ~1e-15 of
 // sqrt(-2 log u1).  Products and sums are written with explicit fma so the result does not
 // depend on floating-point contraction flags.
 #pragma once
 
 #ifndef BM_HD
-#define BM_HD __host__ __device__
+#define BM_HD __device__
 #endif
 
 namespace tip {
@@ -34,7 +44,12 @@ BM_HD inline double bm_log(double x)
     m = lo ? m * 2.0 : m;     // exact
     e = lo ? e - 1 : e;
     const double f = m - 1.0;  // exact (Sterbenz)
-    const double s = f / (m + 1.0);
+    const double d = m + 1.0;
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    y = fma(y, fma(-d, y, 1.0), y);
+    const double q = f * y;
+    const double s = fma(fma(-d, q, f), y, q);
     const double s2 = s * s;
     // 1 + s^2/3 + s^4/5 + ... + s^20/21, Horner in s^2
     double p = 1.0 / 21.0;
@@ -55,6 +70,40 @@ BM_HD inline double bm_log(double x)
     const double ln2_lo = 1.90821492927058770002e-10;
     const double de = (double)e;
     return fma(de, ln2_hi, fma(de, ln2_lo, logm));
+}
+
+// sqrt(t) for a positive normal t below 2^64
+BM_HD inline double bm_sqrt(double t)
+{
+    const double y = __builtin_amdgcn_rsq(t);
+    double g = t * y;
+    double h = 0.5 * y;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    g = fma(fma(-g, g, t), h, g);
+    return fma(fma(-g, g, t), h, g);
+}
+
+// cos(2 pi u) for u in [0, 1)
+BM_HD inline double bm_cos2pi(double u)
+{
+    const double v = u <= 0.5 ? u : 1.0 - u;
+    const double x = 3.141592653589793 * v;  // pi v, in [0, pi/2]
+    const double x2 = x * x;
+    // sin x = x - x^3 p, p = 1/3! - x^2/5! + x^4/7! - ... - x^18/21!, Horner in x^2
+    double p = -1.0 / 51090942171709440000.0;   // -1/21!
+    p = fma(p, x2, 1.0 / 121645100408832000.0);  // 1/19!
+    p = fma(p, x2, -1.0 / 355687428096000.0);    // -1/17!
+    p = fma(p, x2, 1.0 / 1307674368000.0);       // 1/15!
+    p = fma(p, x2, -1.0 / 6227020800.0);         // -1/13!
+    p = fma(p, x2, 1.0 / 39916800.0);            // 1/11!
+    p = fma(p, x2, -1.0 / 362880.0);             // -1/9!
+    p = fma(p, x2, 1.0 / 5040.0);                // 1/7!
+    p = fma(p, x2, -1.0 / 120.0);                // -1/5!
+    p = fma(p, x2, 1.0 / 6.0);                   // 1/3!, subtracted below
+    const double sn = fma(-(x * x2), p, x);
+    return fma(-2.0 * sn, sn, 1.0);
 }
 
 // cos and sin of x in [0, 2^20)

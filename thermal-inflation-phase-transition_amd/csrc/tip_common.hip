@@ -241,11 +241,14 @@ __global__ void k_hash_pairs(const uint64_t* __restrict__ seeds, int64_t n, uint
     if (u1o) u1o[i] = u1;
     if (u2o) u2o[i] = u2;
     if (zo || zso) {
-        double r = sqrt(-2.0 * bm_log(u1));
-        double sn, cs;
-        bm_sincos(6.283185307179586 * u2, &sn, &cs);
-        if (zo) zo[i] = r * cs;
-        if (zso) zso[i] = r * sn;
+        // z: the real field's draw (cos from bm_cos2pi); zs: the pair's second member
+        const double r = bm_sqrt(-2.0 * bm_log(u1));
+        if (zo) zo[i] = r * bm_cos2pi(u2);
+        if (zso) {
+            double sn, cs;
+            bm_sincos(6.283185307179586 * u2, &sn, &cs);
+            zso[i] = r * sn;
+        }
     }
 }
 
@@ -273,11 +276,16 @@ __global__ void k_noise_field(R* __restrict__ o1, R* __restrict__ o2, int64_t n,
     hash_states((uint64_t)(s * mult) ^ noise_seed, x1, x2);
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
-    double r = sqrt(-2.0 * bm_log(u1));
-    double sn, cs;
-    bm_sincos(6.283185307179586 * u2, &sn, &cs);
-    o1[s] = (R)(scale * (r * cs));
-    if (o2) o2[s] = (R)(scale * (r * sn));
+    // the fused steps' draws: one (bm_cos2pi) for the real field, the (cos, sin) pair for two
+    const double r = bm_sqrt(-2.0 * bm_log(u1));
+    if (o2) {
+        double sn, cs;
+        bm_sincos(6.283185307179586 * u2, &sn, &cs);
+        o1[s] = (R)(scale * (r * cs));
+        o2[s] = (R)(scale * (r * sn));
+    } else {
+        o1[s] = (R)(scale * (r * bm_cos2pi(u2)));
+    }
 }
 
 template <typename R>

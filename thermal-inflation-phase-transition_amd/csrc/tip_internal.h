@@ -92,17 +92,17 @@ __host__ __device__ __forceinline__ uint64_t site_seed(int64_t site, uint64_t no
 }
 
 // float64 Box-Muller as written in the reference (cos for the first component, sin for the
-// second of the complex pair, latticeSimComplex_numba.py:2965-2974), with the log and the
-// cos / sin of bm_math.h on the draw's bounded domains (within a few ulp of libm).
+// second of the complex pair, latticeSimComplex_numba.py:2965-2974), with the log, sqrt and
+// cos / sin of bm_math.h on the draw's bounded domains (within a few ulp of libm).  The real
+// field's single draw takes cos(2 pi u2) from one polynomial (bm_cos2pi); the pair takes both
+// from bm_sincos.
 __device__ __forceinline__ double gauss_f64(uint64_t seed)
 {
     uint64_t x1, x2;
     hash_states(seed, x1, x2);
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
-    double s, c;
-    bm_sincos(6.283185307179586 * u2, &s, &c);
-    return sqrt(-2.0 * bm_log(u1)) * c;
+    return bm_sqrt(-2.0 * bm_log(u1)) * bm_cos2pi(u2);
 }
 
 __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double& zs)
@@ -111,7 +111,7 @@ __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double
     hash_states(seed, x1, x2);
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
-    double r = sqrt(-2.0 * bm_log(u1));
+    double r = bm_sqrt(-2.0 * bm_log(u1));
     double s, c;
     bm_sincos(6.283185307179586 * u2, &s, &c);
     zc = r * c;
