@@ -5,7 +5,7 @@
 // Payne-Hanek reduction, sqrt and division scale for denormals and overflow); here the
 // arguments are bounded, so each function is a short, branch-free sequence:
 //
-//  * log(x), x a positive normal number: x = 2^e m with m in [sqrt(1/2), sqrt(2)),
+//  * -2 log(x), x a positive normal number: x = 2^e m with m in [sqrt(1/2), sqrt(2)),
 //    s = (m - 1) / (m + 1), log m = 2 atanh(s) = 2 s (1 + s^2/3 + s^4/5 + ...) (the atanh
 //    series; |s| <= 0.1716, the terms to s^22 leave < 1e-18 relative), and
 //    log x = e ln2_hi + (e ln2_lo + log m) with ln 2 split so that e ln2_hi is exact.
@@ -36,10 +36,12 @@
 
 namespace tip {
 
-BM_HD inline double bm_log(double x)
+// -2 log(x) for a positive normal x (the Box-Muller radius squared); the factor -2 is carried
+// in the constants, which changes no rounding (powers of two)
+BM_HD inline double bm_neg2log(double x)
 {
-    int e;
-    double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    double m = __builtin_amdgcn_frexp_mant(x);  // x = m 2^e, m in [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
     const bool lo = m < 0.70710678118654752440;
     m = lo ? m * 2.0 : m;     // exact
     e = lo ? e - 1 : e;
@@ -62,14 +64,14 @@ BM_HD inline double bm_log(double x)
     p = fma(p, s2, 1.0 / 7.0);
     p = fma(p, s2, 1.0 / 5.0);
     p = fma(p, s2, 1.0 / 3.0);
-    const double t = s * s2;                 // s^3
-    const double logm = fma(2.0 * t, p, 2.0 * s);  // 2 s + 2 s^3 p
+    const double m4s = -4.0 * s;
+    const double neg2logm = fma(m4s * s2, p, m4s);  // -2 (2 s + 2 s^3 p)
     // ln 2 = ln2_hi + ln2_lo, ln2_hi with its low 21 bits zero: e * ln2_hi is exact for
     // |e| < 2^21
-    const double ln2_hi = 6.93147180369123816490e-01;
-    const double ln2_lo = 1.90821492927058770002e-10;
+    const double m2ln2_hi = -2.0 * 6.93147180369123816490e-01;
+    const double m2ln2_lo = -2.0 * 1.90821492927058770002e-10;
     const double de = (double)e;
-    return fma(de, ln2_hi, fma(de, ln2_lo, logm));
+    return fma(de, m2ln2_hi, fma(de, m2ln2_lo, neg2logm));
 }
 
 // sqrt(t) for a positive normal t below 2^64

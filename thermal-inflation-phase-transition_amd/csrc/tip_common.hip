@@ -242,7 +242,7 @@ __global__ void k_hash_pairs(const uint64_t* __restrict__ seeds, int64_t n, uint
     if (u2o) u2o[i] = u2;
     if (zo || zso) {
         // z: the real field's draw (cos from bm_cos2pi); zs: the pair's second member
-        const double r = bm_sqrt(-2.0 * bm_log(u1));
+        const double r = bm_sqrt(bm_neg2log(u1));
         if (zo) zo[i] = r * bm_cos2pi(u2);
         if (zso) {
             double sn, cs;
@@ -277,7 +277,7 @@ __global__ void k_noise_field(R* __restrict__ o1, R* __restrict__ o2, int64_t n,
     double u1 = hash_unit(x1), u2 = hash_unit(x2);
     if (u1 < 1e-12) u1 = 1e-12;
     // the fused steps' draws: one (bm_cos2pi) for the real field, the (cos, sin) pair for two
-    const double r = bm_sqrt(-2.0 * bm_log(u1));
+    const double r = bm_sqrt(bm_neg2log(u1));
     if (o2) {
         double sn, cs;
         bm_sincos(6.283185307179586 * u2, &sn, &cs);

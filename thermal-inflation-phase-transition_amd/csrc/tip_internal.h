@@ -100,18 +100,16 @@ __device__ __forceinline__ double gauss_f64(uint64_t seed)
 {
     uint64_t x1, x2;
     hash_states(seed, x1, x2);
-    double u1 = hash_unit(x1), u2 = hash_unit(x2);
-    if (u1 < 1e-12) u1 = 1e-12;
-    return bm_sqrt(-2.0 * bm_log(u1)) * bm_cos2pi(u2);
+    const double u1 = fmax(hash_unit(x1), 1e-12), u2 = hash_unit(x2);
+    return bm_sqrt(bm_neg2log(u1)) * bm_cos2pi(u2);
 }
 
 __device__ __forceinline__ void gauss_pair_f64(uint64_t seed, double& zc, double& zs)
 {
     uint64_t x1, x2;
     hash_states(seed, x1, x2);
-    double u1 = hash_unit(x1), u2 = hash_unit(x2);
-    if (u1 < 1e-12) u1 = 1e-12;
-    double r = bm_sqrt(-2.0 * bm_log(u1));
+    const double u1 = fmax(hash_unit(x1), 1e-12), u2 = hash_unit(x2);
+    double r = bm_sqrt(bm_neg2log(u1));
     double s, c;
     bm_sincos(6.283185307179586 * u2, &s, &c);
     zc = r * c;
@@ -209,7 +207,8 @@ __device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, ui
 //
 // TableF64: the reference lookup verbatim (latticeSimeRescale_numba.py:1923-1930): truncating
 // int(), index clamped to [0, npts-2], frac unclamped (linear extrapolation).  odd != 0 reads
-// a table tabulated on x >= 0 at |x| and restores the sign.
+// a table tabulated on x >= 0 at |x| and gives the result the sign of x (bit operations on the
+// sign bit: a zero takes the sign of x's zero).
 //
 // TableF32: the same piecewise-linear function stored per segment as (A_i, B_i) with
 // V'(x) = A_i + B_i x, prepared in float64 from the float64 table.  Evaluating in this form
@@ -222,15 +221,18 @@ struct TableF64 {
     int odd;
     // fetch() issues the two table loads of x's segment, eval() finishes the interpolation
     // (recomputing the index), so a caller can issue several sites' loads before it needs them.
+    __device__ __forceinline__ uint64_t sign_mask() const
+    {
+        return odd ? 0x8000000000000000ull : 0ull;
+    }
     __device__ __forceinline__ void index(double x, double& fidx, int& idx) const
     {
-        if (odd && x < 0.0) x = -x;
-        fidx = (x - tmin) * dinv;
-        // Clamping fidx to [-1, last+1] before the conversion changes idx only where the
-        // index clamp below would anyway, and keeps the conversion in range (NaN -> 0).
-        idx = (int)fmin(fmax(fidx, -1.0), (double)last + 1.0);
-        if (idx < 0) idx = 0;
-        else if (idx > last) idx = last;
+        const double ax =
+            __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) & ~sign_mask());
+        fidx = (ax - tmin) * dinv;
+        // int() truncates toward zero, so clamping fidx to [0, last] first gives the
+        // reference's clamp of the truncated index (and 0 for NaN)
+        idx = (int)fmin(fmax(fidx, 0.0), (double)last);
     }
     __device__ __forceinline__ double2 fetch(double x) const
     {
@@ -241,13 +243,13 @@ struct TableF64 {
     }
     __device__ __forceinline__ double eval(double2 s, double x) const
     {
-        const double sgn = (odd && x < 0.0) ? -1.0 : 1.0;
         double fidx;
         int idx;
         index(x, fidx, idx);
         const double frac = fidx - (double)idx;
         const double v = s.x + frac * (s.y - s.x);
-        return odd ? sgn * v : v;
+        return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) ^
+                                              (__builtin_bit_cast(uint64_t, x) & sign_mask()));
     }
     __device__ __forceinline__ double operator()(double x) const { return eval(fetch(x), x); }
 };
