@@ -9,12 +9,12 @@
 //    s = (m - 1) / (m + 1), log m = 2 atanh(s) = 2 s (1 + s^2/3 + s^4/5 + ...) (the atanh
 //    series; |s| <= 0.1716, the terms to s^22 leave < 1e-18 relative), and
 //    log x = e ln2_hi + (e ln2_lo + log m) with ln 2 split so that e ln2_hi is exact.
-//    The quotient is f * y with y the hardware reciprocal of m + 1 after two Newton steps,
-//    then one residual correction q + (f - d q) y (Markstein's division; d in [1.7, 2.5]
-//    needs none of the generic routine's scaling).
+//    The quotient is f * y with y the hardware reciprocal of m + 1 after one Newton step,
+//    then one residual correction q + (f - d q) y, which leaves the square of y's error
+//    (Markstein's division; d in [1.7, 2.5] needs none of the generic routine's scaling).
 //  * sqrt(t), 0 < t < 2^64 (normal): y = hardware 1/sqrt(t), one coupled Newton step for
-//    g ~ sqrt(t) and h ~ 1/(2 sqrt(t)) (Goldschmidt), then two residual corrections
-//    g + (t - g^2) h.
+//    g ~ sqrt(t) and h ~ 1/(2 sqrt(t)) (Goldschmidt; it squares the relative error), then one
+//    residual correction g + (t - g^2) h.
 //  * cos(2 pi u) for u in [0, 1): with v = min(u, 1 - u) (exact for a u of 53 bits),
 //    cos(2 pi u) = 1 - 2 sin^2(pi v) and pi v in [0, pi/2], sin by its Taylor series to the
 //    x^21 term (the next term is below 1.2e-18 there).  For v = u, pi v is exactly half of
@@ -49,7 +49,6 @@ BM_HD inline double bm_neg2log(double x)
     const double d = m + 1.0;
     double y = __builtin_amdgcn_rcp(d);
     y = fma(y, fma(-d, y, 1.0), y);
-    y = fma(y, fma(-d, y, 1.0), y);
     const double q = f * y;
     const double s = fma(fma(-d, q, f), y, q);
     const double s2 = s * s;
@@ -83,7 +82,6 @@ BM_HD inline double bm_sqrt(double t)
     const double e = fma(-h, g, 0.5);
     g = fma(g, e, g);
     h = fma(h, e, h);
-    g = fma(fma(-g, g, t), h, g);
     return fma(fma(-g, g, t), h, g);
 }
 
