@@ -452,8 +452,10 @@ int ready(tip_ensemble* E)
 
 // Steps per captured graph: bounds a graph's node count (groups x steps + 1) whatever the
 // length of the call; an even count leaves the buffer parity unchanged, so one graph per
-// parity serves every full chunk.
-constexpr int64_t kGraphChunk = 16;
+// parity serves every full chunk.  64 steps (a few hundred kernel nodes per chain at the
+// nucleation ensemble's group sizes) make replay as fast as direct launches; 16 steps cost
+// ~5 % in graph launches.
+constexpr int64_t kGraphChunk = 64;
 
 // The graph of k steps of one chain of groups starting at buffer parity `par`, captured on
 // first use: with `chains` = 2 chain c holds groups c, c + 2, ... (the groups the batched
