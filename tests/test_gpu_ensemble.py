@@ -131,11 +131,11 @@ def test_grouped_schedule_identical(group_bytes, concurrency):
         ens_b.set_concurrency(concurrency)
         ens_b.step(2, mode=mode)
         ens_b.step(3, mode=mode)
-        ens_b.step(19, mode=mode)  # a full 16-step graph chunk and a remainder
+        ens_b.step(70, mode=mode)  # a full 64-step graph chunk and a remainder
         ens_a2, _, _, _ = _setup()
-        ens_a2.step(24, mode="batched")
+        ens_a2.step(75, mode="batched")
         assert _same(_state(ens_b), _state(ens_a2)), (group_bytes, concurrency, mode)
-        assert ens_b.step_count == 24
+        assert ens_b.step_count == 75
         ens_b.close()
         ens_a2.close()
 
@@ -236,7 +236,7 @@ def test_graph_after_table_param_and_geometry_changes():
         both(lambda e: e.set_table(i, te, temin, tedinv, odd=False))
     step(4)
     both(lambda e: e.set_geometry(1, 5, 1, 0, 1))
-    step(37)  # chunks of 16, 16 and 5 steps
-    assert ens_g.step_count == ens_b.step_count == 49
+    step(133)  # chunks of 64, 64 and 5 steps
+    assert ens_g.step_count == ens_b.step_count == 145
     ens_b.close()
     ens_g.close()
