@@ -864,7 +864,9 @@ __device__ __forceinline__ float complex_rho(float a, float b)
     return __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
 }
 
-template <typename Radial>
+// ZN = false compiles the Z_N term out (launches with it switched off): a loop body a third
+// of the size.
+template <typename Radial, bool ZN = true>
 __device__ __forceinline__ void complex_force_finish(const Radial& rad, const ZnParams<float>& zn,
                                                      float a, float b, float rho,
                                                      const typename Radial::Pre& pre, float& d1,
@@ -874,7 +876,7 @@ __device__ __forceinline__ void complex_force_finish(const Radial& rad, const Zn
     const float w = rad.eval(pre, rho) * inv;
     d1 = w * a;
     d2 = w * b;
-    if (zn.active && rho > 1e-20f) {
+    if (ZN && zn.active && rho > 1e-20f) {
         float sn;
         if (zn.ipow > 0) {
             const float ca = a * inv, sb = b * inv;
@@ -894,15 +896,114 @@ __device__ __forceinline__ void complex_force_finish(const Radial& rad, const Zn
     }
 }
 
-template <typename Radial>
+template <typename Radial, bool ZN = true>
 __device__ __forceinline__ void complex_force_f32(const Radial& rad, const ZnParams<float>& zn,
                                                   float a, float b, float& d1, float& d2)
 {
     const float rho = complex_rho(a, b);
-    complex_force_finish(rad, zn, a, b, rho, rad.fetch(rho), d1, d2);
+    complex_force_finish<Radial, ZN>(rad, zn, a, b, rho, rad.fetch(rho), d1, d2);
 }
 
-template <int TY, int NM, typename Radial, bool CNT, bool HS = false>
+// The segment table read through the vector cache or from a per-block LDS copy, chosen per
+// block at the start of each launch.  A field spread over many segments within a wave (walls
+// between the false and the true vacuum during the roll, string cores) costs the vector L1 one
+// tag lookup per distinct cache line of a gather, which LDS reads do not pay; a narrow field is
+// served from L1 without the copy.  The choice looks at the block's first plane: if the table
+// indices of any wave there span more than kSpread segments, the block copies the table (at
+// most kMaxSeg segments, 16-byte aligned) into LDS beside the staging area and reads it there.
+// Both paths return the same entry.  With one 8-wave block per CU (four rows per thread, shared
+// halo rows) the 64 KiB costs no waves.
+struct TableRadialAuto {
+    TableF32<false> t;
+    const float2* lt;     // the block's copy (LDS)
+    mutable int use_lds;  // block-uniform
+    using Pre = float2;
+    static constexpr bool kEarly = true;
+    static constexpr bool kBatch = false;
+    static constexpr int kMaxSeg = 8192;
+    static constexpr int kLdsFloats = 2 * kMaxSeg;
+    static constexpr int kSpread = 128;  // segments: 8 cache lines of 16 entries
+    __device__ __forceinline__ void block_init(float* lds)
+    {
+        lt = reinterpret_cast<const float2*>(lds);
+        use_lds = 0;
+    }
+    __device__ __forceinline__ uint32_t index(float rho) const
+    {
+        return (uint32_t)(int)__builtin_amdgcn_fmed3f(fmaf(rho, t.dinv, t.off), 0.0f,
+                                                      (float)t.last);
+    }
+    // all threads of the block, with the block's first plane (phi1, phi2 rows) loaded
+    template <int TY>
+    __device__ __forceinline__ void stage(const float (&a)[TY], const float (&b)[TY]) const
+    {
+        int lo = t.last, hi = 0;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const int i = (int)index(complex_rho(a[r], b[r]));
+            lo = min(lo, i);
+            hi = max(hi, i);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        use_lds = __builtin_amdgcn_readfirstlane(__syncthreads_or(hi - lo > kSpread));
+        if (use_lds) {
+            const int n2 = t.last + 1, n4 = n2 >> 1;
+            const int nt = (int)(blockDim.x * blockDim.y);
+            const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+            const float4* src = reinterpret_cast<const float4*>(t.ab);
+            float4* dst = reinterpret_cast<float4*>(const_cast<float2*>(lt));
+#pragma unroll 8
+            for (int i = tid; i < n4; i += nt) dst[i] = src[i];
+            if ((n2 & 1) && tid == 0) const_cast<float2*>(lt)[n2 - 1] = t.ab[n2 - 1];
+            __syncthreads();
+        }
+    }
+    // the tile runs one of two compiled plane loops per block, so the test is not per gather
+    template <bool LDS>
+    __device__ __forceinline__ Pre fetch_from(float rho) const
+    {
+        const uint32_t i = index(rho);
+        if constexpr (LDS) {
+            return lt[i];
+        } else {
+            const __amdgpu_buffer_rsrc_t T = make_rsrc(t.ab, (uint32_t)(t.last + 1) * 8u);
+            using f2 = float __attribute__((ext_vector_type(2)));
+            const f2 v = __builtin_bit_cast(
+                f2, __builtin_amdgcn_raw_buffer_load_b64(T, (int)(i * 8u), 0, 0));
+            return make_float2(v.x, v.y);
+        }
+    }
+    __device__ __forceinline__ Pre fetch(float rho) const
+    {
+        return use_lds ? fetch_from<true>(rho) : fetch_from<false>(rho);
+    }
+    __device__ __forceinline__ float eval(const Pre& s, float rho) const
+    {
+        return seg_eval<false>(s, rho);
+    }
+};
+
+// radial forms that prepare per-block data inside the tile, after its first loads
+template <typename Radial>
+struct radial_stages : std::false_type {};
+template <>
+struct radial_stages<TableRadialAuto> : std::true_type {};
+
+// the radial's fetch inside the tile's plane loop compiled for one table source (L)
+template <typename Radial, typename L>
+__device__ __forceinline__ typename Radial::Pre rad_fetch(const Radial& rad, float rho, L)
+{
+    if constexpr (radial_stages<Radial>::value)
+        return rad.template fetch_from<L::value>(rho);
+    else
+        return rad.fetch(rho);
+}
+
+template <int TY, int NM, typename Radial, bool CNT, bool HS = false, bool ZN = true>
 __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
@@ -981,9 +1082,11 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             hA[NF + f] = buf_load_s(at(F[f], x0), vo, sop);
         }
     }
+    if constexpr (radial_stages<Radial>::value) rad.stage(v1[0], v1[1]);
     unsigned cnt = 0;
 
-    auto plane_step = [&](auto bufc, int x, const float (&vm)[NF][TY], const float (&vc)[NF][TY],
+    auto plane_step = [&](auto ldsc, auto bufc, int x, const float (&vm)[NF][TY],
+                          const float (&vc)[NF][TY],
                           const float (&vp)[NF][TY], float (&vn)[NF][TY],
                           const float (&hc)[NH], float (&hn)[NH]) {
         constexpr int b = decltype(bufc)::value;
@@ -996,10 +1099,10 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
                 rho1[r] = complex_rho(vc[0][r], vc[1][r]);
-                P1[r] = rad.fetch(rho1[r]);
+                P1[r] = rad_fetch(rad, rho1[r], ldsc);
                 rho2[r] = complex_rho(fmaf(c.half_dt, vc[2][r], vc[0][r]),
                                       fmaf(c.half_dt, vc[3][r], vc[1][r]));
-                P2[r] = rad.fetch(rho2[r]);
+                P2[r] = rad_fetch(rad, rho2[r], ldsc);
             }
         }
 #pragma unroll
@@ -1064,9 +1167,9 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
                                   g2);
             float d1, d2;
             if constexpr (Radial::kEarly || Radial::kBatch)
-                complex_force_finish(rad, zn, a, bb, rho1[r], P1[r], d1, d2);
+                complex_force_finish<Radial, ZN>(rad, zn, a, bb, rho1[r], P1[r], d1, d2);
             else
-                complex_force_f32(rad, zn, a, bb, d1, d2);
+                complex_force_f32<Radial, ZN>(rad, zn, a, bb, d1, d2);
             const float k1a = fmaf(-c.inv_mu2, d1, fmaf(c.c_lap, t1a, -c.eta * u));
             const float k1b = fmaf(-c.inv_mu2, d2, fmaf(c.c_lap, t1b, -c.eta * v));
             const float ga = fmaf(c.half_dt, k1a, u), gb = fmaf(c.half_dt, k1b, v);
@@ -1074,9 +1177,9 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
             const float t2b = fmaf(c.half_dt, fmaf(-6.0f, v, sum[3]), t1b);
             float e1, e2;
             if constexpr (Radial::kEarly || Radial::kBatch)
-                complex_force_finish(rad, zn, fa, fb, rho2[r], P2[r], e1, e2);
+                complex_force_finish<Radial, ZN>(rad, zn, fa, fb, rho2[r], P2[r], e1, e2);
             else
-                complex_force_f32(rad, zn, fa, fb, e1, e2);
+                complex_force_f32<Radial, ZN>(rad, zn, fa, fb, e1, e2);
             const float k2a = fmaf(-c.inv_mu2, e1, fmaf(c.c_lap, t2a, -c.eta * ga));
             const float k2b = fmaf(-c.inv_mu2, e2, fmaf(c.c_lap, t2b, -c.eta * gb));
             const float na = fmaf(c.dt, ga, a), nb = fmaf(c.dt, gb, bb);
@@ -1101,13 +1204,6 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
     };
     using B0 = std::integral_constant<int, 0>;
     using B1 = std::integral_constant<int, 1>;
-    int x = x0;
-    for (; x + 4 <= x1; x += 4) {
-        plane_step(B0{}, x + 0, v0, v1, v2, v3, hA, hB);
-        plane_step(B1{}, x + 1, v1, v2, v3, v0, hB, hA);
-        plane_step(B0{}, x + 2, v2, v3, v0, v1, hA, hB);
-        plane_step(B1{}, x + 3, v3, v0, v1, v2, hB, hA);
-    }
     auto rotate = [&]() {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
@@ -1118,15 +1214,32 @@ __device__ __forceinline__ unsigned rk2_complex_rows_tile(
 #pragma unroll
         for (int k = 0; k < NH; ++k) hA[k] = hB[k];
     };
-    if (x < x1) { plane_step(B0{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
-    if (x < x1) { plane_step(B1{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
-    if (x < x1) plane_step(B0{}, x, v0, v1, v2, v3, hA, hB);
+    auto march = [&](auto ldsc) {
+        int x = x0;
+        for (; x + 4 <= x1; x += 4) {
+            plane_step(ldsc, B0{}, x + 0, v0, v1, v2, v3, hA, hB);
+            plane_step(ldsc, B1{}, x + 1, v1, v2, v3, v0, hB, hA);
+            plane_step(ldsc, B0{}, x + 2, v2, v3, v0, v1, hA, hB);
+            plane_step(ldsc, B1{}, x + 3, v3, v0, v1, v2, hB, hA);
+        }
+        if (x < x1) { plane_step(ldsc, B0{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
+        if (x < x1) { plane_step(ldsc, B1{}, x, v0, v1, v2, v3, hA, hB); rotate(); ++x; }
+        if (x < x1) plane_step(ldsc, B0{}, x, v0, v1, v2, v3, hA, hB);
+    };
+    if constexpr (radial_stages<Radial>::value) {
+        if (rad.use_lds)  // block-uniform
+            march(std::true_type{});
+        else
+            march(std::false_type{});
+    } else {
+        march(std::false_type{});
+    }
     return cnt;
 }
 
 // CNT: also count the sites with |phi_new|^2 > esc2 into *vac (separate instantiation).
 // HS: two row groups per block that share their inner y-halo rows through LDS.
-template <int TY, int NM, typename Radial, bool CNT, bool HS = false>
+template <int TY, int NM, typename Radial, bool CNT, bool HS = false, bool ZN = true>
 __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     const float* __restrict__ phi1, const float* __restrict__ phi2, const float* __restrict__ pi1,
     const float* __restrict__ pi2, float* __restrict__ o1, float* __restrict__ o2,
@@ -1143,7 +1256,7 @@ __global__ __launch_bounds__(512) void k_rk2_complex_rows(
     g.decode((int)blockIdx.x, (int)gridDim.x, tp, ch);
     const int x0 = ch * g.cx;
     const int x1 = min(x0 + g.cx, nx);
-    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT, HS>(
+    unsigned cnt = rk2_complex_rows_tile<TY, NM, Radial, CNT, HS, ZN>(
         phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, r, zn, nx, ny, nz, W, tp, x0,
         x1, noise_seed, esc2, lds_crows);
     if (CNT && cnt && (threadIdx.x & 63) == 0) atomicAdd(vac, (unsigned long long)cnt);

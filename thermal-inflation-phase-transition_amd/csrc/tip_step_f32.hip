@@ -300,15 +300,38 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             const dim3 blk(64, rs.W * rs.G);
             TableRadial t32;
             t32.t = tab.as<false>();
+            // the shared-halo form with four rows per thread (one block per CU) chooses per
+            // block between the vector-cache table and an LDS copy, when the table fits
+            TableRadialAuto t32a;
+            t32a.t = t32.t;
+            t32a.lt = nullptr;
+            t32a.use_lds = 0;
+            const bool auto_tab = hs && rs.TY == 4 && tab.last + 1 <= TableRadialAuto::kMaxSeg &&
+                                  ((uintptr_t)tab.ab & 15) == 0;
+            const size_t lds_a = lds + (size_t)TableRadialAuto::kLdsFloats * sizeof(float);
 #define TIP_LAUNCH_CROWS_H(TYV, NM, CNT, HSV)                                                 \
     hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT, HSV>), dim3(nbr), blk, lds, \
                        st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn,      \
                        (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
-#define TIP_LAUNCH_CROWS_C(TYV, NM, CNT)          \
-    do {                                          \
-        if (hs)                                   \
+#define TIP_LAUNCH_CROWS_AZ(NM, CNT, ZNV)                                                      \
+    hipLaunchKernelGGL((k_rk2_complex_rows<4, NM, TableRadialAuto, CNT, true, ZNV>), dim3(nbr),   \
+                       blk, lds_a, st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c,   \
+                       t32a, zn, (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+// the Z_N term compiled out of launches that switch it off (a third of the loop code)
+#define TIP_LAUNCH_CROWS_A(NM, CNT)                 \
+    do {                                            \
+        if (zn.active)                              \
+            TIP_LAUNCH_CROWS_AZ(NM, CNT, true);     \
+        else                                        \
+            TIP_LAUNCH_CROWS_AZ(NM, CNT, false);    \
+    } while (0)
+#define TIP_LAUNCH_CROWS_C(TYV, NM, CNT)            \
+    do {                                            \
+        if (TYV == 4 && auto_tab)                   \
+            TIP_LAUNCH_CROWS_A(NM, CNT);            \
+        else if (hs)                                \
             TIP_LAUNCH_CROWS_H(TYV, NM, CNT, true); \
-        else                                      \
+        else                                        \
             TIP_LAUNCH_CROWS_H(TYV, NM, CNT, false);\
     } while (0)
 #define TIP_LAUNCH_CROWS(TYV, NM)              \
@@ -333,6 +356,8 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
 #undef TIP_LAUNCH_CROWS
 #undef TIP_LAUNCH_CROWS_C
 #undef TIP_LAUNCH_CROWS_H
+#undef TIP_LAUNCH_CROWS_A
+#undef TIP_LAUNCH_CROWS_AZ
             TIP_HIP(hipGetLastError());
             return TIP_OK;
         }
