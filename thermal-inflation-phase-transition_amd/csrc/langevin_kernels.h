@@ -241,7 +241,8 @@ __global__ __launch_bounds__(64 * BY) void k_langevin_complex(
 // two halo rows, x neighbours the register window (x-1, x, x+1, and x+2 in flight).  Loads
 // per site: phi at x+2, pi at x+1 (BAOAB) and 2/TY halo rows per field.
 // ---------------------------------------------------------------------------------------
-template <int NF, int TY, int OP, bool ODD, bool HS = false>
+// ZN = false: the Z_N term compiled out (complex launches that switch it off)
+template <int NF, int TY, int OP, bool ODD, bool HS = false, bool ZN = true>
 __device__ __forceinline__ void lv_rows_tile(
     const float* const (&phi)[NF], const float* const (&pi)[NF], float* const (&phi_out)[NF],
     float* const (&pi_out)[NF], const LangevinScalars<float>& c, const TableF32<ODD>& tab,
@@ -390,7 +391,8 @@ __device__ __forceinline__ void lv_rows_tile(
             if constexpr (NF == 1) {
                 dV[0] = seg_eval<ODD>(seg_load<ODD>(T, tab, pc[0][r]), pc[0][r]);
             } else {
-                complex_force_finish(rad, zn, pc[0][r], pc[NF - 1][r], rho[r], seg[r], dV[0],
+                complex_force_finish<TableRadial, ZN>(rad, zn, pc[0][r], pc[NF - 1][r], rho[r],
+                                                      seg[r], dV[0],
                                      dV[NF - 1]);
             }
 #pragma unroll
@@ -454,7 +456,7 @@ __device__ __forceinline__ void lv_rows_tile(
     if (x < x1) plane_step(B0{}, x, w0, w1, w2, w3, m0, m1, hA, hB);
 }
 
-template <int NF, int TY, int OP, bool ODD, bool HS = false>
+template <int NF, int TY, int OP, bool ODD, bool HS = false, bool ZN = true>
 __global__ __launch_bounds__(NF == 1 ? 1024 : 512) void k_langevin_rows(
     const float* __restrict__ f0, const float* __restrict__ f1, const float* q0,
     const float* q1, float* __restrict__ o0, float* __restrict__ o1, float* r0, float* r1,
@@ -478,8 +480,8 @@ __global__ __launch_bounds__(NF == 1 ? 1024 : 512) void k_langevin_rows(
         const float* const pi[2] = {q0, q1};
         float* const po[2] = {o0, o1};
         float* const qo[2] = {r0, r1};
-        lv_rows_tile<2, TY, OP, ODD, HS>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0,
-                                          x1, noise_seed, hash, lds_lv);
+        lv_rows_tile<2, TY, OP, ODD, HS, ZN>(phi, pi, po, qo, c, tab, zn, nx, ny, nz, W, tp, x0,
+                                              x1, noise_seed, hash, lds_lv);
     }
 }
 

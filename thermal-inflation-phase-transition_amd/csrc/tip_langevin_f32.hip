@@ -41,11 +41,18 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
     tg.xcd_remap = ((geom ? geom->xcd : 1) && nb % 8 == 0) ? 1 : 0;
     const size_t lds = (size_t)rs.G * (64 * rs.W) * (2 * NF * rs.TY + 1) * sizeof(float);
     const dim3 blk(64, rs.W * rs.G);
-#define TIP_LV_ROWS_H(TYV, OPV, HSV)                                                           \
-    hipLaunchKernelGGL((k_langevin_rows<NF, TYV, OPV, ODD, NF == 2 && HSV>), dim3(nb), blk, lds, \
-                       st, f0, f1,                                                                  \
-                       q0, q1, o0, o1, r0, r1, c, tab, zn, (int)nx, (int)ny, (int)nz, rs.W, tg,    \
-                       noise_seed, hash)
+#define TIP_LV_ROWS_Z(TYV, OPV, HSV, ZNV)                                                     \
+    hipLaunchKernelGGL((k_langevin_rows<NF, TYV, OPV, ODD, NF == 2 && HSV, NF == 1 || ZNV>),     \
+                       dim3(nb), blk, lds, st, f0, f1, q0, q1, o0, o1, r0, r1, c, tab, zn,        \
+                       (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, hash)
+// the Z_N term compiled out of complex launches that switch it off
+#define TIP_LV_ROWS_H(TYV, OPV, HSV)                 \
+    do {                                             \
+        if (NF == 2 && !zn.active)                   \
+            TIP_LV_ROWS_Z(TYV, OPV, HSV, false);     \
+        else                                         \
+            TIP_LV_ROWS_Z(TYV, OPV, HSV, true);      \
+    } while (0)
 #define TIP_LV_ROWS(TYV, OPV)                 \
     do {                                      \
         if (hs)                               \
@@ -69,6 +76,7 @@ int launch_lv_rows(int op, const float* f0, const float* f1, const float* q0, co
 #undef TIP_LV_ROWS_OP
 #undef TIP_LV_ROWS
 #undef TIP_LV_ROWS_H
+#undef TIP_LV_ROWS_Z
     TIP_HIP(hipGetLastError());
     launched = true;
     return TIP_OK;
