@@ -309,10 +309,17 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
             const bool auto_tab = hs && rs.TY == 4 && tab.last + 1 <= TableRadialAuto::kMaxSeg &&
                                   ((uintptr_t)tab.ab & 15) == 0;
             const size_t lds_a = lds + (size_t)TableRadialAuto::kLdsFloats * sizeof(float);
-#define TIP_LAUNCH_CROWS_H(TYV, NM, CNT, HSV)                                                 \
-    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT, HSV>), dim3(nbr), blk, lds, \
-                       st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn,      \
+#define TIP_LAUNCH_CROWS_HZ(TYV, NM, CNT, HSV, ZNV)                                             \
+    hipLaunchKernelGGL((k_rk2_complex_rows<TYV, NM, TableRadial, CNT, HSV, ZNV>), dim3(nbr), blk, \
+                       lds, st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c, t32, zn, \
                        (int)nx, (int)ny, (int)nz, rs.W, tg, noise_seed, vac, esc2)
+#define TIP_LAUNCH_CROWS_H(TYV, NM, CNT, HSV)            \
+    do {                                                 \
+        if (zn.active)                                   \
+            TIP_LAUNCH_CROWS_HZ(TYV, NM, CNT, HSV, true);  \
+        else                                             \
+            TIP_LAUNCH_CROWS_HZ(TYV, NM, CNT, HSV, false); \
+    } while (0)
 #define TIP_LAUNCH_CROWS_AZ(NM, CNT, ZNV)                                                      \
     hipLaunchKernelGGL((k_rk2_complex_rows<4, NM, TableRadialAuto, CNT, true, ZNV>), dim3(nbr),   \
                        blk, lds_a, st, phi1, phi2, pi1, pi2, o1, o2, o3, o4, noise1, noise2, c,   \
@@ -356,6 +363,7 @@ int launch_complex_f32(const float* phi1, const float* phi2, const float* pi1, c
 #undef TIP_LAUNCH_CROWS
 #undef TIP_LAUNCH_CROWS_C
 #undef TIP_LAUNCH_CROWS_H
+#undef TIP_LAUNCH_CROWS_HZ
 #undef TIP_LAUNCH_CROWS_A
 #undef TIP_LAUNCH_CROWS_AZ
             TIP_HIP(hipGetLastError());
