@@ -9,7 +9,7 @@ when |dT|/T > 1e-4 or the field leaves the range, latticeSimeRescale_numba.py:36
 the float64 potential the network was trained on (series J + Coleman-Weinberg).  "surrogate":
 the shipped network inline at the exact T(t), no tables.
 
-Bar (north_star): the true-vacuum fraction and the string count agree within the ensembles'
+Bar (the surrogate's acceptance): the true-vacuum fraction and the string count agree within the ensembles'
 combined 1 sigma at every report step; the run covers the transition (f_true goes from 0 to 1
 and the string count falls by more than 100x).
 """
