@@ -106,13 +106,13 @@ def test_real_drop_in_matches_oracle(shape, op, dtype):
         assert np.array_equal(pi, pi0)
 
 
+@pytest.mark.parametrize("zn", [(3.0, 1e8, 1.0), (3.0, 1e8, 0.0)])
 @pytest.mark.parametrize("op", ["overdamped", "baoab"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_complex_drop_in_matches_oracle(op, dtype):
+def test_complex_drop_in_matches_oracle(op, dtype, zn):
     from tip import kernels_complex as KC
     shape = (10, 9, 70)
     f, q, tab, prm = complex_setup(shape, 5)
-    zn = (3.0, 1e8, 1.0)
     a, b = (x.astype(dtype) for x in f)
     u, v = (x.astype(dtype) for x in q)
     f0 = [a.astype(np.float64), b.astype(np.float64)]
@@ -365,12 +365,14 @@ def test_lattice_integrator_geometries(complex_field):
             assert rel(o[k], ref[k]) < 1e-6, (geo, op, k)
 
 
+@pytest.mark.parametrize("zn_active", [1.0, 0.0])
 @pytest.mark.parametrize("shape", [(6, 16, 250), (5, 12, 256), (4, 10, 128)])
-def test_complex_shared_halo_rows(shape):
+def test_complex_shared_halo_rows(shape, zn_active):
     """Complex rows of two or four waves run two row groups per block that read their inner
     y-halo rows from each other's LDS staging (where ny divides into pairs of tiles): every
     row geometry gives identical BAOAB and overdamped fields, the flat tile agrees to float32
-    rounding, and the default form matches the oracle."""
+    rounding, and the default form matches the oracle; Z_N on, and off (the row kernels with
+    the Z_N term compiled out)."""
     from tip.lattice import Lattice, KERNEL_FLAT, KERNEL_ROWS
     f, q, tab, prm = complex_setup(shape, sum(shape))
     init = {k: v.astype(np.float32) for k, v in
@@ -389,7 +391,7 @@ def test_complex_shared_halo_rows(shape):
                 for k, v in init.items():
                     lat.upload(k, v)
                 lat.set_table(*tab, odd=False)
-                lat.set_zn(3.0, 1e8, 1.0)
+                lat.set_zn(3.0, 1e8, zn_active)
                 lat.set_geometry(*geo)
                 if op == "baoab":
                     lat.step_baoab(4, gamma=prm["eta"], **kw)
@@ -413,8 +415,8 @@ def test_complex_shared_halo_rows(shape):
     for n in range(3):
         KC.overdamped_euler_step_table(a, b, prm["dt"], prm["dx"], prm["eta"], prm["mu"],
                                        prm["inv_a2"], tab[0], tab[1], tab[2], len(tab[0]),
-                                       prm["ns"], O.noise_seed(n), zn=(3.0, 1e8, 1.0))
-    (ra, rb), _ = oracle_complex("overdamped", f0, q0, tab, prm, (3.0, 1e8, 1.0), 3)
+                                       prm["ns"], O.noise_seed(n), zn=(3.0, 1e8, zn_active))
+    (ra, rb), _ = oracle_complex("overdamped", f0, q0, tab, prm, (3.0, 1e8, zn_active), 3)
     assert rel(a.astype(np.float64), ra) < 2e-5 and rel(b.astype(np.float64), rb) < 2e-5
 
 

@@ -400,9 +400,12 @@ def test_c32_vs_oracle_mixed():
         assert rel(gi, ai) < 1e-5
 
 
-def test_complex_lattice_geometries():
+@pytest.mark.parametrize("zn_active", [1.0, 0.0])
+def test_complex_lattice_geometries(zn_active):
     """Complex float32: every row-tile geometry gives identical fields (same per-site
-    arithmetic); the flat tile (the generic formulation) agrees to float32 rounding."""
+    arithmetic); the flat tile (the generic formulation) agrees to float32 rounding.  Z_N on,
+    and off (the row kernels with the Z_N term compiled out, including the per-block table
+    choice of four-row tiles with shared halo rows)."""
     from tip import potential as P
     from tip.lattice import Lattice
     rng = np.random.default_rng(21)
@@ -418,7 +421,7 @@ def test_complex_lattice_geometries():
         for k, v in init.items():
             lat.upload(k, v)
         lat.set_table(t, tmin, dinv)
-        lat.set_zn(3.0, 1e8, 1.0)
+        lat.set_zn(3.0, 1e8, zn_active)
         lat.set_geometry(*geo)
         lat.step(5, seed=4, **kw)
         outs[geo] = [lat.download(k) for k in ("phi1", "phi2", "pi1", "pi2")]
@@ -432,13 +435,14 @@ def test_complex_lattice_geometries():
             assert all(np.array_equal(a, b) for a, b in zip(o, ref)), geo
 
 
+@pytest.mark.parametrize("zn_active", [1.0, 0.0])
 @pytest.mark.parametrize("shape", [(6, 16, 250), (5, 12, 256), (4, 10, 128)])
-def test_complex_shared_halo_rows(shape):
+def test_complex_shared_halo_rows(shape, zn_active):
     """Rows of two or four waves run two row groups per block that read their inner y-halo
     rows from each other's LDS staging (one, two and four rows per thread, x chunks that do
     and do not divide nx, ny that does and does not divide into pairs of tiles): identical
     fields to the unshared form's per-site arithmetic, the flat tile within float32
-    rounding; Z_N on."""
+    rounding; Z_N on, and off (the Z_N term compiled out)."""
     from tip import potential as P
     from tip.lattice import Lattice, KERNEL_FLAT, KERNEL_ROWS
     rng = np.random.default_rng(sum(shape))
@@ -457,7 +461,7 @@ def test_complex_shared_halo_rows(shape):
             for k, v in init.items():
                 lat.upload(k, v)
             lat.set_table(t, tmin, dinv)
-            lat.set_zn(3.0, 1e8, 1.0)
+            lat.set_zn(3.0, 1e8, zn_active)
             lat.set_geometry(*geo)
             lat.step(4, seed=8, step0=0, **kw)
             outs[geo] = [lat.download(k) for k in ("phi1", "phi2", "pi1", "pi2")]
